@@ -1,0 +1,106 @@
+"""Host wrapper of the fused device-resident solver (bbpd_solver_* in include/bbpd.h).
+
+A `Solver` is built from a BeamCase (reference-layout input arrays) and runs whole explicit steps on
+one GPU.  The step sequence restates solvers/scripts/dynamic/dynamicsolverdisplacementcontrolled.m:71-121
+(Euler-Cromer) or its ADR variant (calculatedampingcoefficient.m + timeintegrationADR.m).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .beam import BeamCase, to_problem
+
+
+class Solver:
+    def __init__(self, case: BeamCase, precision: int = _lib.FP64, device: int = 0):
+        L = _lib.lib()
+        if L.bbpd_device_count() <= device:
+            raise _lib.BBPDError("no HIP device: the fused solver runs on the GPU only")
+        self.case = case
+        self.precision = precision
+        keep: list = []
+        prob = to_problem(case, keep)
+        h = C.c_void_p()
+        _lib.check(L.bbpd_solver_create(C.byref(prob), precision, device, C.byref(h)), "bbpd_solver_create")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bbpd_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- running
+    def step(self, n: int = 1):
+        _lib.check(self._L.bbpd_solver_step(self._h, n), "bbpd_solver_step")
+
+    def run_graph(self, n: int, reps: int = 1):
+        _lib.check(self._L.bbpd_solver_run_graph(self._h, n, reps), "bbpd_solver_run_graph")
+
+    def sync(self):
+        _lib.check(self._L.bbpd_solver_sync(self._h), "bbpd_solver_sync")
+
+    @property
+    def stream(self) -> int:
+        return self._L.bbpd_solver_stream(self._h) or 0
+
+    # ---- results
+    def observables(self) -> dict:
+        o = _lib.Observables()
+        _lib.check(self._L.bbpd_solver_observables(self._h, C.byref(o)), "bbpd_solver_observables")
+        return {f: getattr(o, f) for f, _ in o._fields_}
+
+    def nodal(self):
+        """(u, v, F, x) in the reference layout: (nNodes, 3) Fortran-order float64."""
+        nN = self.case.nNodes
+        arrs = [np.zeros((nN, 3), order="F") for _ in range(4)]
+        p = [a.ctypes.data_as(C.c_void_p) for a in arrs]
+        _lib.check(self._L.bbpd_solver_get_nodal(self._h, *p), "bbpd_solver_get_nodal")
+        return tuple(arrs)
+
+    def bonds(self):
+        """(bondSofteningFactor, flagBondSoftening, fail) per bond in BONDLIST order."""
+        nB = self.case.nBonds
+        arrs = [np.zeros(nB) for _ in range(3)]
+        p = [a.ctypes.data_as(C.c_void_p) for a in arrs]
+        _lib.check(self._L.bbpd_solver_get_bonds(self._h, *p), "bbpd_solver_get_bonds")
+        return tuple(arrs)
+
+    def history(self, cap: int = 1 << 20):
+        """(reaction, deflection) of every step since the previous call."""
+        r = np.zeros(cap)
+        d = np.zeros(cap)
+        n = C.c_int()
+        _lib.check(self._L.bbpd_solver_history(self._h, r.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p),
+                                               cap, C.byref(n)), "bbpd_solver_history")
+        return r[:n.value].copy(), d[:n.value].copy()
+
+    def info(self) -> dict:
+        v = [C.c_int64() for _ in range(4)]
+        _lib.check(self._L.bbpd_solver_info(self._h, *[C.byref(x) for x in v]), "bbpd_solver_info")
+        return dict(device_bytes=v[0].value, bond_kernel_bytes=v[1].value, n_entries=v[2].value,
+                    n_padded_entries=v[3].value)
+
+    def set_timing(self, on: bool):
+        _lib.check(self._L.bbpd_solver_set_timing(self._h, 1 if on else 0), "bbpd_solver_set_timing")
+
+    def kernel_time(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        _lib.check(self._L.bbpd_solver_kernel_time(self._h, C.byref(ms), C.byref(n)), "bbpd_solver_kernel_time")
+        return ms.value, n.value

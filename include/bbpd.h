@@ -1,0 +1,235 @@
+/*
+ * bbpd.h -- C ABI of the MI355X bond-based peridynamics step library (libbbpd_hip.so).
+ *
+ * Two layers:
+ *
+ *  1. Drop-in routines.  One entry point per computational routine of the reference's MEX
+ *     files and per MATLAB-only hot-path function, with the reference's argument order, array
+ *     layout (FP64, column-major, BONDLIST as nBonds x 2 doubles of 1-based ids, flags as doubles)
+ *     and in-place semantics.  Arrays are DEVICE pointers (hipMalloc'd or torch CUDA tensors);
+ *     `stream` is a hipStream_t (NULL = default stream).  Results are identical bit for bit to the
+ *     reference C routines where those use only +,-,*,/,sqrt (everything except exp()).
+ *     A MEX gateway binds each routine exactly as the reference's gateway does; the gateway code
+ *     for every name is given in INTEGRATION.md.
+ *
+ *  2. Fused solver.  A device-resident state built once from the same reference arrays, that runs
+ *     whole steps (stretch -> law -> force -> node-centric gather -> update -> contact) with no
+ *     per-step host traffic, in plain FP64 (bit-identical to the reference order) or FP32-mixed.
+ *
+ * Every call returns BBPD_OK (0) or a negative code; bbpd_last_error() gives the message (the
+ * reference reports only a wrong argument count, through mexErrMsgTxt).
+ */
+#ifndef BBPD_H
+#define BBPD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBPD_OK 0
+#define BBPD_ERR_ARG (-1)
+#define BBPD_ERR_HIP (-2)
+#define BBPD_ERR_NOMEM (-3)
+#define BBPD_ERR_UNSUPPORTED (-4)
+
+/* constitutive laws of the fused solver */
+#define BBPD_LAW_LINEAR 0      /* brittle latch, calculatebondfailureMex.c:26-44 */
+#define BBPD_LAW_BILINEAR 1    /* calculatebondsofteningfactor.m:42-48 */
+#define BBPD_LAW_TRILINEAR 2   /* calculatebsftrilinear.m:40-112 */
+#define BBPD_LAW_EXPONENTIAL 3 /* calculatebsfexponentialMex.c:26-66 */
+
+#define BBPD_INTEGRATOR_EULER_CROMER 0 /* timeintegrationeulercromerMex.c:26-53 */
+#define BBPD_INTEGRATOR_ADR 1          /* calculatedampingcoefficient.m + timeintegrationADR.m */
+
+#define BBPD_PRECISION_FP64 0
+#define BBPD_PRECISION_MIXED 1
+
+const char *bbpd_last_error(void);
+int bbpd_version(void);
+/* number of visible HIP devices (0 when none); never fails */
+int bbpd_device_count(void);
+
+/* ============================================================================================
+ * Layer 1: drop-in routines (device pointers)
+ * ========================================================================================== */
+
+/* replaces calculatedeformedlength, solvers/functions/mex/calculatedeformedlengthMex.c:27 */
+int bbpd_calculatedeformedlength(const double *deformedCoordinates, const double *BONDLIST,
+                                 const double *undeformedLength, int nBonds, int nNodes, double *deformedLength,
+                                 double *deformedX, double *deformedY, double *deformedZ, double *stretch,
+                                 void *stream);
+
+/* replaces calculatebsfexponential, solvers/functions/mex/calculatebsfexponentialMex.c:26 */
+int bbpd_calculatebsfexponential(const double *stretch, const double *s0, const double *sc, double k, double alpha,
+                                 double *bsf, double *bondSofteningFactor, const double *BONDTYPE,
+                                 double *flagBondSoftening, int nBonds, void *stream);
+
+/* replaces calculatebondforces, solvers/functions/mex/calculatebondforcesMex.c:26 */
+int bbpd_calculatebondforces(double *bForceX, double *bForceY, double *bForceZ, const double *fail,
+                             const double *deformedX, const double *deformedY, const double *deformedZ,
+                             const double *deformedLength, const double *stretch, const double *stretchPlastic,
+                             const double *BONDSTIFFNESS, double cellVolume, const double *VOLUMECORRECTIONFACTORS,
+                             const double *bondSofteningFactor, int nBonds, void *stream);
+
+/* replaces calculatenodalforces of solvers/functions/mex/calculatenodalforcesopenmpMex.c:29
+ * (nThreads = 8 reproduces that file, which forces 8 threads at :39) and of the serial
+ * solvers/functions/mex/calculatenodalforcesMex.c:19 (nThreads = 1, nfShared* may be NULL).
+ * nodalForce and the nNodes x nThreads scratch arrays must be zeroed by the caller, as in the
+ * reference; the scratch columns receive the per-thread partial sums the reference leaves there.
+ * The summation order reproduces the reference's for that thread count, with no atomics. */
+int bbpd_calculatenodalforces(int nBonds, int nNodes, const double *BONDLIST, double *nodalForce,
+                              const double *bForceX, const double *bForceY, const double *bForceZ, double *nfSharedX,
+                              double *nfSharedY, double *nfSharedZ, int nThreads, void *stream);
+
+/* replaces timeintegrationeulercromer, solvers/functions/mex/timeintegrationeulercromerMex.c:26 */
+int bbpd_timeintegrationeulercromer(double *nodalAcceleration, const double *nodalForce, double DAMPING,
+                                    double *nodalVelocity, const double *DENSITY, const double *constraintFlag,
+                                    double DT, double *nodalDisplacement, double *deformedCoordinates,
+                                    const double *undeformedCoordinates, int nNodes, int NOD, void *stream);
+
+/* replaces calculatebondfailure, solvers/functions/mex/calculatebondfailureMex.c:26 */
+int bbpd_calculatebondfailure(double *fail, double failureFunctionality, const double *BONDTYPE,
+                              const double *stretch, double criticalStretchConcrete, double criticalStretchSteel,
+                              int nBonds, void *stream);
+
+/* replaces solvers/functions/calculatebondsofteningfactor.m:1 (bilinear law).  s0/sc are arrays
+ * with stride 1 or scalars (stride 0, pointer to one device double), as MATLAB broadcasts. */
+int bbpd_calculatebondsofteningfactor(const double *stretch, const double *linearElasticLimit, int s0_stride,
+                                      const double *criticalStretchConcrete, int sc_stride,
+                                      double *flagBondSoftening, double *bondSofteningFactor,
+                                      const double *BONDTYPE, int nBonds, void *stream);
+
+/* replaces solvers/functions/calculatebsftrilinear.m:1 (scalar s0, s1, sc as the .m requires) */
+int bbpd_calculatebsftrilinear(const double *stretch, double s0, double s1, double sc, double *bondSofteningFactor,
+                               const double *BONDTYPE, double *flagBondSoftening, int nBonds, void *stream);
+
+/* replaces solvers/functions/calculatedampingcoefficient.m:1 (NOD = 3).  Writes cn to *cn_host
+ * (host memory) and synchronises the stream.  The two sums use a fixed-order device reduction. */
+int bbpd_calculatedampingcoefficient(const double *nodalForce, const double *massVector,
+                                     const double *nodalForcePrevious, double DT,
+                                     const double *nodalVelocityPreviousHalf, const double *nodalDisplacement,
+                                     int nNodes, double *cn_host, void *stream);
+
+/* replaces solvers/functions/timeintegrationADR.m:1 (NOD = 3) */
+int bbpd_timeintegrationADR(int iTimeStep, double DT, double *nodalVelocityForwardHalf, const double *nodalForce,
+                            const double *massVector, double cn, const double *nodalVelocityPreviousHalf,
+                            double *nodalDisplacement, const double *CONSTRAINTFLAG, int nNodes, void *stream);
+
+/* replaces solvers/functions/calculatecontactforce.m:1.  family holds nFamily 1-based node ids
+ * (doubles, device); centre (cx, cz), radius as in the penetrator struct.  Writes penetratorfz to
+ * *fz_host and synchronises the stream. */
+int bbpd_calculatecontactforce(const double *family, int nFamily, double centreX, double centreZ, double radius,
+                               double displacementIncrement, const double *undeformedCoordinates,
+                               double *deformedCoordinates, double *nodalDisplacement, double *nodalVelocity,
+                               double DT, double cellVolume, const double *DENSITY, int nNodes, double *fz_host,
+                               void *stream);
+
+/* ============================================================================================
+ * Input generator (host memory; no GPU needed)
+ * ========================================================================================== */
+typedef struct bbpd_beam_spec {
+    int nx, ny, nz;         /* nodes along x (length), y (width), z (depth) */
+    double DX;              /* lattice spacing (m) */
+    double horizon_factor;  /* delta = horizon_factor * DX */
+    int notch;              /* 1: remove bonds crossing the notch rectangle */
+    double notch_x;         /* plane x = notch_x * DX (half-integer) */
+    double notch_depth;     /* notch top z = notch_depth * DX */
+    double E;               /* Young's modulus (Pa) */
+} bbpd_beam_spec;
+
+int bbpd_beam_count(const bbpd_beam_spec *spec, int64_t *nNodes, int64_t *nBonds);
+/* X: nNodes x 3; BONDLIST: nBonds x 2; others nBonds (nFAMILYMEMBERS: nNodes), all host */
+int bbpd_beam_build(const bbpd_beam_spec *spec, int64_t nBondsCap, double *X, double *BONDLIST,
+                    double *UNDEFORMEDLENGTH, double *VOLUMECORRECTIONFACTORS, double *BONDSTIFFNESS,
+                    double *nFAMILYMEMBERS);
+
+/* ============================================================================================
+ * Layer 2: fused device-resident solver
+ * ========================================================================================== */
+#define BBPD_MAX_BODIES 8
+
+typedef struct bbpd_problem {
+    int nNodes, nBonds;
+    /* reference input arrays (HOST memory, reference layout) */
+    const double *undeformedCoordinates;   /* nNodes x 3 */
+    const double *BONDLIST;                /* nBonds x 2, 1-based */
+    const double *UNDEFORMEDLENGTH;        /* nBonds; must equal the length recomputed from X */
+    const double *VOLUMECORRECTIONFACTORS; /* nBonds; must equal calculatevolumecorrectionfactors(L0) */
+    const double *BONDSTIFFNESS;           /* nBonds */
+    const double *BONDTYPE;                /* nBonds or NULL (all concrete) */
+    const double *fail;                    /* nBonds or NULL (all 1) */
+    const double *DENSITY;                 /* nNodes */
+    const double *CONSTRAINTFLAG;          /* nNodes x 3 or NULL */
+    const double *massVector;              /* nNodes (ADR) or NULL */
+    double cellVolume, horizon, RADIJ;     /* RADIJ as in calculatevolumecorrectionfactors */
+    /* law */
+    int law;
+    const double *s0;  /* nBonds or NULL -> s0_scalar */
+    const double *sc;  /* nBonds or NULL -> sc_scalar */
+    double s0_scalar, s1, sc_scalar, k, alpha;
+    double sc_steel, failureFunctionality; /* linear law */
+    /* integration */
+    int integrator;
+    double DT, DAMPING;
+    /* contact bodies (calculatecontactforce.m), body 0 gives the reaction */
+    int nBodies;
+    const double *family[BBPD_MAX_BODIES]; /* 1-based ids, host */
+    int nFamily[BBPD_MAX_BODIES];
+    double centreX[BBPD_MAX_BODIES], centreZ[BBPD_MAX_BODIES], radius[BBPD_MAX_BODIES];
+    int moves[BBPD_MAX_BODIES]; /* 1: centre follows the load schedule */
+    /* load schedule: 0 = smoothstepdata(i, 1, nTimeSteps, 0, appliedDisplacement);
+     * 1 = staircase: -increment * ceil(i / itersPerIncrement) signed by appliedDisplacement */
+    int schedule;
+    int nTimeSteps;
+    double appliedDisplacement;
+    double increment;
+    int itersPerIncrement;
+    /* observables */
+    int referenceNode; /* 1-based; 0 = none */
+    int cmodNode1, cmodNode2;
+} bbpd_problem;
+
+typedef struct bbpd_solver bbpd_solver;
+
+typedef struct bbpd_observables {
+    int64_t step;          /* steps done */
+    double reaction;       /* body-0 contact reaction of the last step (calculatecontactforce.m:121) */
+    double reaction_static;/* sum of F_z * cellVolume over body-0 family (calculatereactionforceFast.m:3-5) */
+    double deflection;     /* u_z at referenceNode */
+    double cmod;           /* u_x(cmod2) - u_x(cmod1) */
+    double cn;             /* last ADR damping coefficient */
+    int64_t n_softening;   /* sum(flagBondSoftening) over bonds */
+    int64_t n_broken;      /* bonds with bondSofteningFactor == 1 or fail == 0 */
+    int64_t n_failed;      /* nBonds - sum(fail) (printoutput.m:40) */
+} bbpd_observables;
+
+int bbpd_solver_create(const bbpd_problem *problem, int precision, int device, bbpd_solver **out);
+int bbpd_solver_destroy(bbpd_solver *s);
+/* run n steps on the solver's stream (asynchronous) */
+int bbpd_solver_step(bbpd_solver *s, int n);
+/* capture `n` steps as one HIP graph and replay it `reps` times */
+int bbpd_solver_run_graph(bbpd_solver *s, int n, int reps);
+int bbpd_solver_sync(bbpd_solver *s);
+void *bbpd_solver_stream(bbpd_solver *s);
+int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *obs);
+/* state in reference layout, host memory (NULL pointers skipped):
+ * nodal arrays nNodes x 3, bond arrays nBonds in BONDLIST order */
+int bbpd_solver_get_nodal(bbpd_solver *s, double *nodalDisplacement, double *nodalVelocity, double *nodalForce,
+                          double *deformedCoordinates);
+int bbpd_solver_get_bonds(bbpd_solver *s, double *bondSofteningFactor, double *flagBondSoftening, double *fail);
+/* load-point reaction history: the body-0 reaction of every step since the last call */
+int bbpd_solver_history(bbpd_solver *s, double *reaction, double *deflection, int cap, int *n_out);
+/* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */
+int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel_bytes, int64_t *n_entries,
+                     int64_t *n_padded_entries);
+/* average duration (ms) of the bond kernel over the steps since the last call, from HIP events
+ * recorded on the solver's stream around each launch (0 when timing is off) */
+int bbpd_solver_set_timing(bbpd_solver *s, int on);
+int bbpd_solver_kernel_time(bbpd_solver *s, double *avg_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BBPD_H */

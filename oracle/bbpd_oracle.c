@@ -1,0 +1,611 @@
+/*
+ * bbpd_oracle.c -- CPU restatement of the reference bond-based peridynamics time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/, the smoke check and the
+ * CPU-baseline leg of bench.py load it, and only as the checker or the timed CPU baseline.  The
+ * product (bb_pd_amd/) never links or calls it.
+ *
+ * Every routine restates one function of mark-hobbs/BB_PD operation for operation, in the
+ * reference data model: FP64, column-major nodal arrays (x block, then y at +nNodes, then z at
+ * +2*nNodes), BONDLIST as nBonds x 2 doubles holding 1-based node ids, flags as doubles.  The
+ * citation above each routine names the reference file and lines it follows.  Build with
+ * -ffp-contract=off so that no multiply-add is fused: the reference's gcc build on x86-64 does not
+ * fuse either, and the bit patterns of this file are compared with the GPU's.
+ *
+ * Parity pins (what checks each routine without running the reference, which needs MATLAB's
+ * mex.h and so cannot be built in this image) are listed in the header of oracle/oracle.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------------------------------
+ * A  calculatedeformedlength  -- solvers/functions/mex/calculatedeformedlengthMex.c:27-47
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatedeformedlength(const double *x, const double *BONDLIST, const double *L0, int nBonds,
+                                int nNodes, double *l, double *dX, double *dY, double *dZ, double *s)
+{
+    for (int k = 0; k < nBonds; k++) {
+        int i = (int)BONDLIST[k];
+        int j = (int)BONDLIST[nBonds + k];
+        dX[k] = x[j - 1] - x[i - 1];
+        dY[k] = x[j + nNodes - 1] - x[i + nNodes - 1];
+        dZ[k] = x[j + 2 * nNodes - 1] - x[i + 2 * nNodes - 1];
+        l[k] = sqrt(dX[k] * dX[k] + dY[k] * dY[k] + dZ[k] * dZ[k]);
+        s[k] = (l[k] - L0[k]) / L0[k];
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * B4 exponential softening -- solvers/functions/mex/calculatebsfexponentialMex.c:26-66
+ *    bsf is the per-step scratch the caller zeroes (dynamicsolverdisplacementcontrolled.m:86).
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatebsfexponential(const double *s, const double *s0, const double *sc, double k, double alpha,
+                                double *bsf, double *H, const double *BONDTYPE, double *flag, int nBonds)
+{
+    for (int b = 0; b < nBonds; b++) {
+        if (BONDTYPE[b] == 0) {
+            if (s0[b] < s[b] && s[b] <= sc[b]) {
+                flag[b] = 1;
+                double numerator = 1 - exp(-k * (s[b] - s0[b]) / (sc[b] - s0[b]));
+                double denominator = 1 - exp(-k);
+                double residual = alpha * (1 - (s[b] - s0[b]) / (sc[b] - s0[b]));
+                bsf[b] = 1 - (s0[b] / s[b]) * ((1 - (numerator / denominator)) + residual) / (1 + alpha);
+            } else if (s[b] > sc[b]) {
+                bsf[b] = 1;
+            }
+            if (bsf[b] > H[b]) H[b] = bsf[b];
+            if (H[b] > 1) H[b] = 1;
+            if (H[b] != H[b]) H[b] = 0.0;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * B2 bilinear softening -- solvers/functions/calculatebondsofteningfactor.m:42-48
+ *    s0/sc are per-bond arrays when s0_stride/sc_stride is 1, scalars when 0 (MATLAB broadcasting).
+ *    MATLAB's max(a,b) returns the non-NaN operand, so a NaN candidate leaves H unchanged.
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatebondsofteningfactor(const double *s, const double *s0, int s0_stride, const double *sc,
+                                     int sc_stride, double *flag, double *H, const double *BONDTYPE, int nBonds)
+{
+    for (int b = 0; b < nBonds; b++) {
+        double a = s0[(size_t)b * s0_stride];
+        double c = sc[(size_t)b * sc_stride];
+        if (BONDTYPE[b] == 0 && s[b] > a) flag[b] = 1;                   /* :42 */
+        double cand = (((s[b] - a) / s[b]) * (c / (c - a))) * flag[b];    /* :44, :46 (bsf .* flag) */
+        if (cand == cand && cand > H[b]) H[b] = cand;                     /* :46 max ignores NaN */
+        if (H[b] > 1) H[b] = 1;                                           /* :47 */
+        if (H[b] != H[b]) H[b] = 0;                                       /* :48 */
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * B3 trilinear softening -- solvers/functions/calculatebsftrilinear.m:40-112 (scalar s0, s1, sc)
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatebsftrilinear(const double *s, double s0, double s1, double sc, double *H,
+                              const double *BONDTYPE, double *flag, int nBonds)
+{
+    const double beta = 0.25;                                             /* :41 */
+    const double eta = s1 / s0;                                           /* :42 */
+    for (int b = 0; b < nBonds; b++) {
+        double bsf = 0;                                                   /* :44 */
+        if (BONDTYPE[b] == 0) {
+            if ((s[b] > s0) && (s[b] <= s1)) {                            /* :84-87 */
+                bsf = (1 - ((eta - beta) / (eta - 1) * (s0 / s[b]))) + ((1 - beta) / (eta - 1));
+                flag[b] = 1;
+            } else if ((s[b] > s1) && (s[b] <= sc)) {                     /* :89-91 */
+                bsf = 1 - ((s0 * beta) / s[b]) * ((sc - s[b]) / (sc - s1));
+            } else if (s[b] > sc) {                                       /* :93-96 */
+                bsf = 1;
+            }
+            if (bsf > H[b]) H[b] = bsf;                                   /* :100-104 */
+        }
+        if (H[b] != H[b]) H[b] = 0;                                       /* :112 */
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * B1 linear (brittle) failure latch -- solvers/functions/mex/calculatebondfailureMex.c:26-44
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatebondfailure(double *fail, double ff, const double *BONDTYPE, const double *s, double csc,
+                             double css, int nBonds)
+{
+    for (int b = 0; b < nBonds; b++) {
+        if (fail[b] == 1 && BONDTYPE[b] == 0 && s[b] > csc) fail[b] = ff;
+        if (fail[b] == 1 && BONDTYPE[b] == 1 && s[b] > 3 * csc) fail[b] = ff;
+        if (fail[b] == 1 && BONDTYPE[b] == 2 && s[b] > css) fail[b] = ff;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * C  calculatebondforces -- solvers/functions/mex/calculatebondforcesMex.c:26-45
+ *    left-to-right product, NaN -> 0 per component.
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatebondforces(double *fX, double *fY, double *fZ, const double *fail, const double *dX,
+                            const double *dY, const double *dZ, const double *l, const double *s,
+                            const double *sp, const double *c, double V, const double *vcf, const double *H,
+                            int nBonds)
+{
+    for (int b = 0; b < nBonds; b++) {
+        fX[b] = fail[b] * c[b] * (1 - H[b]) * (s[b] - sp[b]) * V * vcf[b] * (dX[b] / l[b]);
+        fY[b] = fail[b] * c[b] * (1 - H[b]) * (s[b] - sp[b]) * V * vcf[b] * (dY[b] / l[b]);
+        fZ[b] = fail[b] * c[b] * (1 - H[b]) * (s[b] - sp[b]) * V * vcf[b] * (dZ[b] / l[b]);
+        if (fX[b] != fX[b]) fX[b] = 0.0;
+        if (fY[b] != fY[b]) fY[b] = 0.0;
+        if (fZ[b] != fZ[b]) fZ[b] = 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * D  nodal force reduction, serial order -- solvers/functions/mex/calculatenodalforcesMex.c:19-46
+ *    F must be zeroed by the caller (dynamicsolverdisplacementcontrolled.m:97).
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatenodalforces_serial(int nBonds, int nNodes, const double *BONDLIST, double *F, const double *fX,
+                                    const double *fY, const double *fZ)
+{
+    for (int k = 0; k < nBonds; k++) {
+        int i = (int)BONDLIST[k];
+        int j = (int)BONDLIST[nBonds + k];
+        F[i - 1] += fX[k];
+        F[j - 1] -= fX[k];
+        F[i + nNodes - 1] += fY[k];
+        F[j + nNodes - 1] -= fY[k];
+        F[i + 2 * nNodes - 1] += fZ[k];
+        F[j + 2 * nNodes - 1] -= fZ[k];
+    }
+}
+
+/* Static-schedule block of thread t for n iterations over T threads (libgomp's schedule(static)
+ * without a chunk: the first n%T threads take one extra iteration). */
+static void static_block(int n, int T, int t, int *lo, int *hi)
+{
+    int q = n / T, r = n % T;
+    *lo = t * q + (t < r ? t : r);
+    *hi = *lo + q + (t < r ? 1 : 0);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * D' nodal force reduction, OpenMP order -- solvers/functions/mex/calculatenodalforcesopenmpMex.c:29-85
+ *    Emulated sequentially: thread t scatters its static block of bonds into column t of the
+ *    nNodes x T scratch (:50-66), then every node adds the columns in thread order (:69-81).  The
+ *    result is what the reference returns with T threads (it forces T = 8, :39).  Scratch arrays
+ *    must be zeroed by the caller (dynamicsolverdisplacementcontrolled.m:98-100).
+ * ---------------------------------------------------------------------------------------------- */
+void or_calculatenodalforces_openmp(int nBonds, int nNodes, const double *BONDLIST, double *F, const double *fX,
+                                    const double *fY, const double *fZ, double *sX, double *sY, double *sZ,
+                                    int T)
+{
+    for (int t = 0; t < T; t++) {
+        int lo, hi;
+        static_block(nBonds, T, t, &lo, &hi);
+        for (int k = lo; k < hi; k++) {
+            int i = (int)BONDLIST[k];
+            int j = (int)BONDLIST[nBonds + k];
+            sX[(i - 1) + (size_t)nNodes * t] += fX[k];
+            sX[(j - 1) + (size_t)nNodes * t] -= fX[k];
+            sY[(i - 1) + (size_t)nNodes * t] += fY[k];
+            sY[(j - 1) + (size_t)nNodes * t] -= fY[k];
+            sZ[(i - 1) + (size_t)nNodes * t] += fZ[k];
+            sZ[(j - 1) + (size_t)nNodes * t] -= fZ[k];
+        }
+    }
+    for (int n = 0; n < nNodes; n++)
+        for (int t = 0; t < T; t++) {
+            F[n] += sX[n + (size_t)nNodes * t];
+            F[n + nNodes] += sY[n + (size_t)nNodes * t];
+            F[n + 2 * nNodes] += sZ[n + (size_t)nNodes * t];
+        }
+}
+
+/* The same reduction executed by a real OpenMP team of T threads, with the loop structure of
+ * calculatenodalforcesopenmpMex.c:41-82 (parallel region, `omp for` over bonds into the thread's
+ * scratch column, `omp for` over nodes adding the columns).  Used to check that the sequential
+ * emulation above reproduces libgomp's static schedule. */
+void or_calculatenodalforces_omp_real(int nBonds, int nNodes, const double *BONDLIST, double *F, const double *fX,
+                                      const double *fY, const double *fZ, double *sX, double *sY, double *sZ, int T)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(T);
+#pragma omp parallel
+    {
+        int nT = omp_get_num_threads();
+        int t = omp_get_thread_num();
+#pragma omp for
+        for (int k = 0; k < nBonds; k++) {
+            int i = (int)BONDLIST[k];
+            int j = (int)BONDLIST[nBonds + k];
+            sX[(i - 1) + (size_t)nNodes * t] += fX[k];
+            sX[(j - 1) + (size_t)nNodes * t] -= fX[k];
+            sY[(i - 1) + (size_t)nNodes * t] += fY[k];
+            sY[(j - 1) + (size_t)nNodes * t] -= fY[k];
+            sZ[(i - 1) + (size_t)nNodes * t] += fZ[k];
+            sZ[(j - 1) + (size_t)nNodes * t] -= fZ[k];
+        }
+#pragma omp for
+        for (int n = 0; n < nNodes; n++)
+            for (int tt = 0; tt < nT; tt++) {
+                F[n] += sX[n + (size_t)nNodes * tt];
+                F[n + nNodes] += sY[n + (size_t)nNodes * tt];
+                F[n + 2 * nNodes] += sZ[n + (size_t)nNodes * tt];
+            }
+    }
+#else
+    or_calculatenodalforces_openmp(nBonds, nNodes, BONDLIST, F, fX, fY, fZ, sX, sY, sZ, T);
+#endif
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * E  Euler-Cromer update -- solvers/functions/mex/timeintegrationeulercromerMex.c:26-53
+ * ---------------------------------------------------------------------------------------------- */
+void or_timeintegrationeulercromer(double *a, const double *F, double D, double *v, const double *rho,
+                                   const double *cons, double DT, double *u, double *x, const double *X,
+                                   int nNodes, int NOD)
+{
+    for (int n = 0; n < nNodes; n++)
+        for (int d = 0; d < NOD; d++) {
+            size_t q = n + (size_t)nNodes * d;
+            a[q] = (F[q] - D * v[q]) / rho[n];
+            if (cons[q] == 1) a[q] = 0;
+            v[q] = v[q] + (a[q] * DT);
+            u[q] = u[q] + (v[q] * DT);
+            x[q] = X[q] + u[q];
+        }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * ADR damping coefficient -- solvers/functions/calculatedampingcoefficient.m:59-112 (NOD == 3)
+ *    Sequential node-major sum, massVector(i,1) for every DOF (:65,:71,:77).
+ * ---------------------------------------------------------------------------------------------- */
+double or_calculatedampingcoefficient(const double *F, const double *m, const double *Fp, double DT,
+                                      const double *vp, const double *u, int nNodes)
+{
+    double cn = 0, cn1 = 0, cn2 = 0;
+    for (int i = 0; i < nNodes; i++) {
+        for (int d = 0; d < 3; d++) {
+            size_t q = i + (size_t)nNodes * d;
+            if (vp[q] != 0)
+                cn1 = cn1 - (u[q] * u[q]) * ((F[q] / m[i] - Fp[q] / m[i]) / (DT * vp[q]));
+        }
+        for (int d = 0; d < 3; d++) {
+            size_t q = i + (size_t)nNodes * d;
+            cn2 = cn2 + u[q] * u[q];
+        }
+    }
+    if (cn2 != 0) {
+        if ((cn1 / cn2) > 0) cn = 2 * sqrt(cn1 / cn2);
+        else cn = 0;
+    } else {
+        cn = 0;
+    }
+    if (cn > 2) cn = 1.9;
+    return cn;
+}
+
+/* Same quantity with the two sums returned, for checking a GPU reduction (a different order). */
+void or_dampingsums(const double *F, const double *m, const double *Fp, double DT, const double *vp,
+                    const double *u, int nNodes, double *cn1_out, double *cn2_out)
+{
+    long double cn1 = 0, cn2 = 0;
+    for (int i = 0; i < nNodes; i++)
+        for (int d = 0; d < 3; d++) {
+            size_t q = i + (size_t)nNodes * d;
+            if (vp[q] != 0) cn1 -= (long double)((u[q] * u[q]) * ((F[q] / m[i] - Fp[q] / m[i]) / (DT * vp[q])));
+            cn2 += (long double)(u[q] * u[q]);
+        }
+    *cn1_out = (double)cn1;
+    *cn2_out = (double)cn2;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * ADR update -- solvers/functions/timeintegrationADR.m:51-80 (NOD == 3); m^-1 is 1/m.
+ * ---------------------------------------------------------------------------------------------- */
+void or_timeintegrationADR(int iTimeStep, double DT, double *vf, const double *F, const double *m, double cn,
+                           const double *vp, double *u, const double *cons, int nNodes)
+{
+    for (int n = 0; n < nNodes; n++) {
+        double minv = 1 / m[n];
+        for (int d = 0; d < 3; d++) {
+            size_t q = n + (size_t)nNodes * d;
+            if (iTimeStep == 1)
+                vf[q] = 0.5 * DT * minv * F[q];
+            else
+                vf[q] = (((2 - cn * DT) * vp[q]) + (2 * DT * minv * F[q])) / (2 + cn * DT);
+            u[q] = u[q] + (DT * vf[q]);
+        }
+    }
+    for (size_t q = 0; q < (size_t)nNodes * 3; q++)
+        if (cons[q] == 1) {
+            u[q] = 0;
+            vf[q] = 0;
+        }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Fictitious ADR mass -- solvers/functions/buildmassmatrix.m:32-40
+ * ---------------------------------------------------------------------------------------------- */
+void or_buildmassmatrix(int nNodes, double DX, double c0, double DT, const double *nFAM, double *m)
+{
+    for (int n = 0; n < nNodes; n++) {
+        double Kij = (nFAM[n] * (DX * DX)) * (c0 / DX);
+        m[n] = 0.25 * (DT * DT) * Kij;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Rigid-cylinder contact -- solvers/functions/calculatecontactforce.m:62-121
+ *    family holds 1-based node ids.  The centre moves by `disp` for this call only (:70; the
+ *    struct is passed by value).  Returns penetratorfz (:121), summed in family order.
+ * ---------------------------------------------------------------------------------------------- */
+double or_calculatecontactforce(const double *family, int nFam, double cx, double cz, double R, double disp,
+                                const double *X, double *x, double *u, double *v, double DT, double V,
+                                const double *rho, int nNodes)
+{
+    double czm = cz + disp;
+    double fz = 0;
+    for (int f = 0; f < nFam; f++) {
+        int i = (int)family[f] - 1;
+        double ddx = x[i] - cx;
+        double ddz = x[i + 2 * (size_t)nNodes] - czm;
+        double dist = sqrt((ddx * ddx) + (ddz * ddz));
+        if (dist < R) {
+            double ux = (ddx / dist) * R, uz = (ddz / dist) * R;
+            x[i] = cx + ux;
+            x[i + 2 * (size_t)nNodes] = czm + uz;
+            double dfz = 0;
+            for (int d = 0; d < 3; d++) {
+                size_t q = i + (size_t)nNodes * d;
+                double uo = u[q], vo = v[q];
+                u[q] = x[q] - X[q];
+                v[q] = (u[q] - uo) / DT;
+                double dF = -1 * rho[i] * (v[q] - vo) / DT * V;
+                if (d == 2) dfz = dF;
+            }
+            fz = fz + dfz;
+        }
+    }
+    return fz;
+}
+
+/* smoothstepdata -- solvers/functions/smoothstepdata.m:40-42 */
+double or_smoothstepdata(double cur, double start, double fin, double a0, double a1)
+{
+    double xi = (cur - start) / (fin - start);
+    return a0 + (a1 - a0) * (xi * xi * xi) * (10 - 15 * xi + 6 * (xi * xi));
+}
+
+/* damage field -- solvers/functions/calculatedamage.m:34-45 (broken = fail == 0 or H == 1 here) */
+void or_calculatedamage(const double *BONDLIST, const double *unbroken, const double *nFAM, int nBonds, int nNodes,
+                        double *damage)
+{
+    double *cnt = (double *)calloc((size_t)nNodes, sizeof(double));
+    for (int k = 0; k < nBonds; k++) {
+        int i = (int)BONDLIST[k], j = (int)BONDLIST[nBonds + k];
+        cnt[i - 1] += unbroken[k];
+        cnt[j - 1] += unbroken[k];
+    }
+    for (int n = 0; n < nNodes; n++) damage[n] = 1 - (cnt[n] / nFAM[n]);
+    free(cnt);
+}
+
+/* dissipated damage energy -- solvers/functions/calculatedamageenergy.m:35-62 */
+void or_calculatedamageenergy(const double *c, const double *s0, const double *sc, const double *L0,
+                              const double *s, double k, double alpha, double V, double *de_hist, int nBonds)
+{
+    double denominator = 2 * k * (exp(k) - 1) * (1 + alpha);
+    double shape = (2 * k) - (2 * exp(k)) + (alpha * k) - (alpha * k * exp(k)) + 2;
+    for (int b = 0; b < nBonds; b++) {
+        double de = 0;
+        if ((s0[b] < s[b]) && (s[b] <= sc[b])) {
+            double numerator = c[b] * s0[b] * L0[b] * (s0[b] - s[b]) * shape;
+            de = (numerator / denominator) * (V * V);
+        } else if (s[b] > sc[b]) {
+            double numerator = c[b] * s0[b] * L0[b] * (s0[b] - sc[b]) * shape;
+            de = (numerator / denominator) * (V * V);
+        }
+        if (de > de_hist[b]) de_hist[b] = de;
+    }
+}
+
+/* ================================================================================================
+ * One explicit step in the reference driver's order
+ * (solvers/scripts/dynamic/dynamicsolverdisplacementcontrolled.m:71-121), ADR variant from
+ * calculatedampingcoefficient.m + timeintegrationADR.m with the commented hand-over lines of
+ * dynamicsolverdisplacementcontrolledADR.m:153-155.  All arrays in reference layout.
+ * ============================================================================================== */
+typedef struct or_state {
+    int nNodes, nBonds;
+    /* inputs */
+    const double *X, *BONDLIST, *L0, *vcf, *c, *BONDTYPE, *s0, *sc, *rho, *cons, *mass;
+    int s0_stride, sc_stride; /* 0 = scalar in s0[0]/sc[0] */
+    double V, DT, DAMPING, s1, k, alpha, sc_steel, ff;
+    int law;        /* 0 linear, 1 bilinear, 2 trilinear, 3 exponential */
+    int integrator; /* 0 Euler-Cromer, 1 ADR */
+    int nodal_threads; /* 0 serial reduction order, T>0 OpenMP order with T threads */
+    /* contact bodies */
+    int nBodies;
+    const double *const *families;
+    const int *nFams;
+    const double *centres; /* nBodies x 2 (cx, cz) */
+    const double *radii;
+    const int *moves; /* 1 if the body follows the load schedule */
+    /* state (in/out) */
+    double *x, *u, *v, *a, *F, *Fp, *H, *flag, *fail;
+    /* scratch */
+    double *l, *dX, *dY, *dZ, *s, *bsf, *fX, *fY, *fZ, *sp, *sXs, *sYs, *sZs;
+    double *s0_full, *sc_full; /* expanded per-bond copies for the exponential routine */
+} or_state;
+
+/* returns the reaction of body 0; disp is the total load-point displacement for this step */
+double or_step(or_state *S, int iTimeStep, double disp, double *fz_bodies)
+{
+    int nN = S->nNodes, nB = S->nBonds;
+    or_calculatedeformedlength(S->x, S->BONDLIST, S->L0, nB, nN, S->l, S->dX, S->dY, S->dZ, S->s);
+    switch (S->law) {
+    case 0:
+        or_calculatebondfailure(S->fail, S->ff, S->BONDTYPE, S->s, S->sc[0], S->sc_steel, nB);
+        break;
+    case 1:
+        or_calculatebondsofteningfactor(S->s, S->s0, S->s0_stride, S->sc, S->sc_stride, S->flag, S->H, S->BONDTYPE, nB);
+        break;
+    case 2:
+        or_calculatebsftrilinear(S->s, S->s0[0], S->s1, S->sc[0], S->H, S->BONDTYPE, S->flag, nB);
+        break;
+    default:
+        memset(S->bsf, 0, sizeof(double) * nB);
+        or_calculatebsfexponential(S->s, S->s0_full, S->sc_full, S->k, S->alpha, S->bsf, S->H, S->BONDTYPE, S->flag, nB);
+        break;
+    }
+    or_calculatebondforces(S->fX, S->fY, S->fZ, S->fail, S->dX, S->dY, S->dZ, S->l, S->s, S->sp, S->c, S->V, S->vcf,
+                           S->H, nB);
+    memset(S->F, 0, sizeof(double) * 3 * nN);
+    if (S->nodal_threads <= 0) {
+        or_calculatenodalforces_serial(nB, nN, S->BONDLIST, S->F, S->fX, S->fY, S->fZ);
+    } else {
+        size_t sz = sizeof(double) * (size_t)nN * S->nodal_threads;
+        memset(S->sXs, 0, sz);
+        memset(S->sYs, 0, sz);
+        memset(S->sZs, 0, sz);
+        or_calculatenodalforces_openmp(nB, nN, S->BONDLIST, S->F, S->fX, S->fY, S->fZ, S->sXs, S->sYs, S->sZs,
+                                       S->nodal_threads);
+    }
+    if (S->integrator == 0) {
+        or_timeintegrationeulercromer(S->a, S->F, S->DAMPING, S->v, S->rho, S->cons, S->DT, S->u, S->x, S->X, nN, 3);
+    } else {
+        double cn = or_calculatedampingcoefficient(S->F, S->mass, S->Fp, S->DT, S->v, S->u, nN);
+        /* v holds the previous half-step velocity on entry and the new one on exit */
+        or_timeintegrationADR(iTimeStep, S->DT, S->v, S->F, S->mass, cn, S->v, S->u, S->cons, nN);
+        for (size_t q = 0; q < (size_t)nN * 3; q++) S->x[q] = S->X[q] + S->u[q];
+    }
+    double r0 = 0;
+    for (int b = 0; b < S->nBodies; b++) {
+        double d = S->moves[b] ? disp : 0;
+        double fz = or_calculatecontactforce(S->families[b], S->nFams[b], S->centres[2 * b], S->centres[2 * b + 1],
+                                             S->radii[b], d, S->X, S->x, S->u, S->v, S->DT, S->V, S->rho, nN);
+        if (fz_bodies) fz_bodies[b] = fz;
+        if (b == 0) r0 = fz;
+    }
+    if (S->integrator == 1) memcpy(S->Fp, S->F, sizeof(double) * 3 * nN);
+    return r0;
+}
+
+/* ================================================================================================
+ * Timed CPU baseline: passes A-E as separate OpenMP loops over nBonds-length intermediates with a
+ * static schedule and the per-thread scratch reduction (calculatenodalforcesopenmpMex.c:41-82).
+ * Same arithmetic as above; used only to time the reference's CPU path.
+ * ============================================================================================== */
+void or_step_omp(or_state *S, int T)
+{
+    int nN = S->nNodes, nB = S->nBonds;
+    const double *x = S->x, *BL = S->BONDLIST, *L0 = S->L0;
+#ifdef _OPENMP
+    omp_set_num_threads(T);
+#endif
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nB; k++) {
+        int i = (int)BL[k], j = (int)BL[nB + k];
+        S->dX[k] = x[j - 1] - x[i - 1];
+        S->dY[k] = x[j + nN - 1] - x[i + nN - 1];
+        S->dZ[k] = x[j + 2 * nN - 1] - x[i + 2 * nN - 1];
+        S->l[k] = sqrt(S->dX[k] * S->dX[k] + S->dY[k] * S->dY[k] + S->dZ[k] * S->dZ[k]);
+        S->s[k] = (S->l[k] - L0[k]) / L0[k];
+    }
+    if (S->law == 1) {
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < nB; b++) {
+            double a = S->s0[(size_t)b * S->s0_stride], c = S->sc[(size_t)b * S->sc_stride];
+            if (S->BONDTYPE[b] == 0 && S->s[b] > a) S->flag[b] = 1;
+            double cand = (((S->s[b] - a) / S->s[b]) * (c / (c - a))) * S->flag[b];
+            if (cand == cand && cand > S->H[b]) S->H[b] = cand;
+            if (S->H[b] > 1) S->H[b] = 1;
+        }
+    } else if (S->law == 3) {
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < nB; b++) S->bsf[b] = 0;
+#pragma omp parallel
+        {
+#ifdef _OPENMP
+            int t = omp_get_thread_num(), TT = omp_get_num_threads();
+#else
+            int t = 0, TT = 1;
+#endif
+            int lo, hi;
+            static_block(nB, TT, t, &lo, &hi);
+            or_calculatebsfexponential(S->s + lo, S->s0_full + lo, S->sc_full + lo, S->k, S->alpha, S->bsf + lo,
+                                       S->H + lo, S->BONDTYPE + lo, S->flag + lo, hi - lo);
+        }
+    } else if (S->law == 2) {
+#pragma omp parallel
+        {
+#ifdef _OPENMP
+            int t = omp_get_thread_num(), TT = omp_get_num_threads();
+#else
+            int t = 0, TT = 1;
+#endif
+            int lo, hi;
+            static_block(nB, TT, t, &lo, &hi);
+            or_calculatebsftrilinear(S->s + lo, S->s0[0], S->s1, S->sc[0], S->H + lo, S->BONDTYPE + lo,
+                                     S->flag + lo, hi - lo);
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nB; b++) {
+        double p = S->fail[b] * S->c[b] * (1 - S->H[b]) * (S->s[b] - S->sp[b]) * S->V * S->vcf[b];
+        S->fX[b] = p * (S->dX[b] / S->l[b]);
+        S->fY[b] = p * (S->dY[b] / S->l[b]);
+        S->fZ[b] = p * (S->dZ[b] / S->l[b]);
+        if (S->fX[b] != S->fX[b]) S->fX[b] = 0.0;
+        if (S->fY[b] != S->fY[b]) S->fY[b] = 0.0;
+        if (S->fZ[b] != S->fZ[b]) S->fZ[b] = 0.0;
+    }
+    size_t nsc = (size_t)nN * T;
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < nsc; q++) {
+        S->sXs[q] = 0;
+        S->sYs[q] = 0;
+        S->sZs[q] = 0;
+    }
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < 3 * nN; q++) S->F[q] = 0;
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+#else
+        int t = 0;
+#endif
+#pragma omp for schedule(static)
+        for (int k = 0; k < nB; k++) {
+            int i = (int)BL[k], j = (int)BL[nB + k];
+            S->sXs[(i - 1) + (size_t)nN * t] += S->fX[k];
+            S->sXs[(j - 1) + (size_t)nN * t] -= S->fX[k];
+            S->sYs[(i - 1) + (size_t)nN * t] += S->fY[k];
+            S->sYs[(j - 1) + (size_t)nN * t] -= S->fY[k];
+            S->sZs[(i - 1) + (size_t)nN * t] += S->fZ[k];
+            S->sZs[(j - 1) + (size_t)nN * t] -= S->fZ[k];
+        }
+#pragma omp for schedule(static)
+        for (int n = 0; n < nN; n++)
+            for (int tt = 0; tt < T; tt++) {
+                S->F[n] += S->sXs[n + (size_t)nN * tt];
+                S->F[n + nN] += S->sYs[n + (size_t)nN * tt];
+                S->F[n + 2 * nN] += S->sZs[n + (size_t)nN * tt];
+            }
+    }
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nN; n++)
+        for (int d = 0; d < 3; d++) {
+            size_t q = n + (size_t)nN * d;
+            S->a[q] = (S->F[q] - S->DAMPING * S->v[q]) / S->rho[n];
+            if (S->cons[q] == 1) S->a[q] = 0;
+            S->v[q] = S->v[q] + (S->a[q] * S->DT);
+            S->u[q] = S->u[q] + (S->v[q] * S->DT);
+            S->x[q] = S->X[q] + S->u[q];
+        }
+}
+
+int or_sizeof_state(void) { return (int)sizeof(or_state); }

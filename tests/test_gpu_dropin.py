@@ -1,0 +1,232 @@
+"""Drop-in device routines (layer 1 of include/bbpd.h) against the oracle, through the C ABI.
+
+Bar: bit-exact for every routine built from +, -, *, /, sqrt (stretch, bond force, both nodal
+reduction orders, Euler-Cromer, bilinear/trilinear/brittle laws, ADR update, contact); the
+exponential law differs from glibc only through exp(), tolerance 1e-14 relative on the softening
+factor with identical flags; the ADR damping coefficient uses a tree sum, tolerance 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from bb_pd_amd import _lib, beam, mex
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64)).view(np.int64)
+
+
+def same(a, b):
+    return np.array_equal(bits(a), bits(b))
+
+
+D = mex.to_device
+H = mex.to_host
+
+
+@pytest.fixture(scope="module")
+def case():
+    print("case: 24x6x10 lattice, dx 5 mm, notched", flush=True)
+    return beam.build_beam("t", 24, 6, 10, 5e-3, law=1, sc=2e-3, notch=(12.5, 3.5))
+
+
+def random_state(case, seed, scale=2e-6):
+    rng = np.random.default_rng(seed)
+    u = np.asfortranarray(rng.standard_normal((case.nNodes, 3)) * scale)
+    return np.asfortranarray(case.X + u), u
+
+
+def test_deformedlength(gpu, oracle, case):
+    x, _ = random_state(case, 1)
+    nB = case.nBonds
+    outs = [np.zeros(nB) for _ in range(5)]
+    oracle.calculatedeformedlength(x, case.BONDLIST, case.L0, *outs)
+    dev = [D(np.zeros(nB)) for _ in range(5)]
+    mex.calculatedeformedlengthMex(D(x), D(case.BONDLIST), D(case.L0), *dev)
+    for o, d in zip(outs, dev):
+        assert same(o, H(d))
+
+
+def test_deformedlength_full_beam(gpu, oracle):
+    c = beam.case_bilinear_dx5()
+    print("case: 100x10x20 beam, 1,005,544 bonds", flush=True)
+    x, _ = random_state(c, 2)
+    nB = c.nBonds
+    outs = [np.zeros(nB) for _ in range(5)]
+    oracle.calculatedeformedlength(x, c.BONDLIST, c.L0, *outs)
+    dev = [D(np.zeros(nB)) for _ in range(5)]
+    mex.calculatedeformedlengthMex(D(x), D(c.BONDLIST), D(c.L0), *dev)
+    for o, d in zip(outs, dev):
+        assert same(o, H(d))
+
+
+def _stretches(n, s0, sc, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.uniform(-2e-3, 3 * sc, n)
+    s[:10] = [s0, sc, 0.0, -0.0, s0 * (1 + 1e-15), sc * (1 - 1e-15), 2 * sc, -s0, 1e-20, s0 / 2]
+    return s
+
+
+def test_exponential_law(gpu, oracle):
+    n, s0, sc = 50000, 1e-4, 3e-3
+    s = _stretches(n, s0, sc, 3)
+    rng = np.random.default_rng(4)
+    s0a = s0 * (1 + 0.01 * rng.uniform(-1, 1, n))
+    sca = np.full(n, sc)
+    BT = rng.choice([0.0, 1.0, 2.0], n, p=[0.8, 0.1, 0.1])
+    H0 = rng.uniform(0, 0.5, n) * (rng.uniform(size=n) < 0.3)
+    fl0 = (rng.uniform(size=n) < 0.2).astype(float)
+    for step in range(2):
+        bsf_o, H_o, fl_o = np.zeros(n), H0.copy(), fl0.copy()
+        oracle.calculatebsfexponential(s, s0a, sca, 25.0, 0.25, bsf_o, H_o, BT, fl_o)
+        bsf_d, H_d, fl_d = D(np.zeros(n)), D(H0), D(fl0)
+        mex.calculatebsfexponentialMex(D(s), D(s0a), D(sca), 25, 0.25, bsf_d, H_d, D(BT), fl_d)
+        assert same(fl_o, H(fl_d))
+        np.testing.assert_allclose(H(H_d), H_o, rtol=1e-14, atol=0)
+        np.testing.assert_allclose(H(bsf_d), bsf_o, rtol=1e-14, atol=0)
+        H0 = H_o
+
+
+def test_bondforces(gpu, oracle):
+    n = 40000
+    rng = np.random.default_rng(5)
+    args = dict(fail=(rng.uniform(size=n) < 0.9).astype(float), dX=rng.standard_normal(n), dY=rng.standard_normal(n),
+                dZ=rng.standard_normal(n), s=rng.standard_normal(n) * 1e-3, sp=rng.standard_normal(n) * 1e-5,
+                c=rng.uniform(1e20, 2e20, n), vcf=rng.uniform(0, 1, n), H=rng.uniform(0, 1, n))
+    l = np.sqrt(args["dX"] ** 2 + args["dY"] ** 2 + args["dZ"] ** 2)
+    l[:5] = 0.0
+    args["dX"][:5] = args["dY"][:5] = args["dZ"][:5] = 0.0
+    V = 1.25e-7
+    fo = [np.zeros(n) for _ in range(3)]
+    oracle.calculatebondforces(*fo, args["fail"], args["dX"], args["dY"], args["dZ"], l, args["s"], args["sp"],
+                               args["c"], V, args["vcf"], args["H"])
+    fd = [D(np.zeros(n)) for _ in range(3)]
+    mex.calculatebondforcesMex(*fd, D(args["fail"]), D(args["dX"]), D(args["dY"]), D(args["dZ"]), D(l), D(args["s"]),
+                               D(args["sp"]), D(args["c"]), V, D(args["vcf"]), D(args["H"]))
+    for o, d in zip(fo, fd):
+        assert same(o, H(d))
+    assert np.all(H(fd[0])[:5] == 0.0)
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_nodal_reduction_both_orders(gpu, oracle, case, shuffle):
+    BL = case.BONDLIST.copy(order="F")
+    rng = np.random.default_rng(6)
+    if shuffle:  # arbitrary bond order and orientation, as MATLAB may hand over
+        BL = np.asfortranarray(BL[rng.permutation(BL.shape[0])])
+        flip = rng.uniform(size=BL.shape[0]) < 0.5
+        BL[flip] = BL[flip][:, ::-1]
+    nN, nB = case.nNodes, BL.shape[0]
+    f = [rng.standard_normal(nB) * 10.0 ** rng.integers(-6, 7, nB) for _ in range(3)]
+    # serial order (calculatenodalforcesMex.c)
+    Fo = np.zeros((nN, 3), order="F")
+    oracle.calculatenodalforces_serial(BL, Fo, *f)
+    Fd = D(np.zeros((nN, 3), order="F"))
+    mex.calculatenodalforcesMex(D(BL), Fd, *[D(a) for a in f])
+    assert same(Fo, H(Fd))
+    # OpenMP order with 8 threads (calculatenodalforcesopenmpMex.c), scratch columns included
+    T = 8
+    Fo8 = np.zeros((nN, 3), order="F")
+    so = [np.zeros(nN * T) for _ in range(3)]
+    oracle.calculatenodalforces_openmp(BL, Fo8, *f, *so, T)
+    Fd8 = D(np.zeros((nN, 3), order="F"))
+    sd = [D(np.zeros((nN, T), order="F")) for _ in range(3)]
+    mex.calculatenodalforcesopenmpMex(D(BL), Fd8, *[D(a) for a in f], *sd)
+    assert same(Fo8, H(Fd8))
+    for a, b in zip(so, sd):
+        assert same(a, H(b).reshape(-1, order="F"))
+    assert not same(Fo, Fo8)      # the two orders really differ on this data
+
+
+def test_eulercromer(gpu, oracle, case):
+    rng = np.random.default_rng(8)
+    nN = case.nNodes
+    F = np.asfortranarray(rng.standard_normal((nN, 3)) * 1e6)
+    v = np.asfortranarray(rng.standard_normal((nN, 3)))
+    u = np.asfortranarray(rng.standard_normal((nN, 3)) * 1e-5)
+    cons = np.asfortranarray((rng.uniform(size=(nN, 3)) < 0.1).astype(float))
+    rho = rng.uniform(2000, 2500, nN)
+    a, x = np.zeros((nN, 3), order="F"), np.zeros((nN, 3), order="F")
+    dv = [D(t) for t in (np.zeros((nN, 3), order="F"), F, v, rho, cons, u, x, case.X)]
+    oracle.timeintegrationeulercromer(a, F, 0.3, v, rho, cons, 1.25e-6, u, x, case.X)
+    mex.timeintegrationeulercromerMex(dv[0], dv[1], 0.3, dv[2], dv[3], dv[4], 1.25e-6, dv[5], dv[6], dv[7])
+    for o, d in zip((a, v, u, x), (dv[0], dv[2], dv[5], dv[6])):
+        assert same(o, H(d))
+
+
+def test_laws_bilinear_trilinear_brittle(gpu, oracle):
+    n, s0, s1, sc = 60000, 1e-4, 5e-4, 3e-3
+    s = _stretches(n, s0, sc, 9)
+    rng = np.random.default_rng(10)
+    BT = rng.choice([0.0, 1.0, 2.0], n, p=[0.8, 0.1, 0.1])
+    H0 = rng.uniform(0, 0.5, n) * (rng.uniform(size=n) < 0.3)
+    fl0 = (rng.uniform(size=n) < 0.2).astype(float)
+    s0a = s0 * (1 + 0.01 * rng.uniform(-1, 1, n))
+    # bilinear with per-bond s0 and scalar sc, then both scalar
+    for s0arg in (s0a, s0):
+        Ho, flo = H0.copy(), fl0.copy()
+        oracle.calculatebondsofteningfactor(s, s0arg, sc, flo, Ho, BT)
+        Hd, fld = D(H0), D(fl0)
+        mex.calculatebondsofteningfactor(D(s), D(s0arg) if isinstance(s0arg, np.ndarray) else s0arg, sc, fld, Hd, D(BT))
+        assert same(Ho, H(Hd)) and same(flo, H(fld))
+    # trilinear
+    Ho, flo = H0.copy(), fl0.copy()
+    oracle.calculatebsftrilinear(s, s0, s1, sc, Ho, BT, flo)
+    Hd, fld = D(H0), D(fl0)
+    mex.calculatebsftrilinear(D(s), s0, s1, sc, Hd, D(BT), fld)
+    assert same(Ho, H(Hd)) and same(flo, H(fld))
+    # brittle latch
+    f0 = (rng.uniform(size=n) < 0.95).astype(float)
+    fo = f0.copy()
+    oracle.calculatebondfailure(fo, 0.0, BT, s, sc, 5e-3)
+    fd = D(f0)
+    mex.calculatebondfailureMex(fd, 0.0, D(BT), D(s), sc, 5e-3)
+    assert same(fo, H(fd))
+
+
+def test_adr_routines(gpu, oracle, case):
+    rng = np.random.default_rng(11)
+    nN = case.nNodes
+    F = np.asfortranarray(rng.standard_normal((nN, 3)))
+    Fp = np.asfortranarray(F + rng.standard_normal((nN, 3)) * 0.1)
+    vp = np.asfortranarray(rng.standard_normal((nN, 3)) * (rng.uniform(size=(nN, 3)) < 0.8))
+    u = np.asfortranarray(rng.standard_normal((nN, 3)))
+    m = rng.uniform(1, 2, nN)
+    cn_o = oracle.calculatedampingcoefficient(F, m, Fp, 1.0, vp, u)
+    cn_d = mex.calculatedampingcoefficient(D(F), D(m), D(Fp), 1.0, D(vp), D(u))
+    assert cn_d == pytest.approx(cn_o, rel=1e-12)
+    cons = np.asfortranarray((rng.uniform(size=(nN, 3)) < 0.1).astype(float))
+    for it in (1, 2):
+        vo, uo = vp.copy(order="F"), u.copy(order="F")
+        oracle.timeintegrationADR(it, 1.0, vo, F, m, 0.7, vo, uo, cons)
+        vd, ud = D(vp), D(u)
+        mex.timeintegrationADR(it, 1.0, vd, D(F), D(m), 0.7, vd, ud, D(cons))
+        assert same(vo, H(vd)) and same(uo, H(ud))
+
+
+def test_contact(gpu, oracle):
+    c = beam.case_bilinear_dx5()
+    rng = np.random.default_rng(12)
+    u = np.asfortranarray(rng.standard_normal((c.nNodes, 3)) * 1e-5)
+    x = np.asfortranarray(c.X + u)
+    v = np.asfortranarray(rng.standard_normal((c.nNodes, 3)) * 1e-3)
+    for b in c.bodies:
+        disp = -3e-5 if b.moves else 0.0
+        xo, uo, vo = x.copy(order="F"), u.copy(order="F"), v.copy(order="F")
+        fz_o = oracle.calculatecontactforce(b.family, b.cx, b.cz, b.R, disp, c.X, xo, uo, vo, 1.2e-6, c.cellVolume,
+                                            c.DENSITY)
+        xd, ud, vd = D(x), D(u), D(v)
+        pen = {"family": D(b.family), "centre": (b.cx, b.cz), "radius": b.R}
+        *_, fz_d = mex.calculatecontactforce(pen, disp, D(c.X), xd, ud, vd, 1.2e-6, c.cellVolume, D(c.DENSITY))
+        assert same(xo, H(xd)) and same(uo, H(ud)) and same(vo, H(vd))
+        assert same([fz_o], [fz_d])
+        if b.moves:
+            assert fz_o != 0.0
+
+
+def test_wrong_argument_count(gpu):
+    with pytest.raises(_lib.BBPDError, match="Eight input arguments required"):
+        mex.calculatedeformedlengthMex(1, 2, 3)
+    with pytest.raises(_lib.BBPDError, match="Fourteen input arguments required"):
+        mex.calculatebondforcesMex()

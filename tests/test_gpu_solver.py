@@ -1,0 +1,194 @@
+"""Fused device-resident solver (layer 2) against the oracle's reference step sequence.
+
+Bars
+  * plain FP64, Euler-Cromer, bilinear / trilinear / brittle laws: the whole trajectory (u, v, x,
+    nodal force, softening factor, flags, fail, reaction) is bit-identical to the oracle's serial
+    reference order, step after step;
+  * exponential law: exp() differs from glibc by ulps, so 1e-10 relative (north-star tolerance for
+    nodal forces) with identical flags on these short runs;
+  * ADR: the damping coefficient is a tree sum on the device and a sequential sum in the reference,
+    so 1e-10 relative on displacements, forces and the reaction;
+  * FP32-mixed: stated error against FP64 on every result, flips of near-threshold bonds counted.
+"""
+import numpy as np
+import pytest
+
+from bb_pd_amd import _lib, beam
+from bb_pd_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64)).view(np.int64)
+
+
+def same(a, b):
+    return np.array_equal(bits(a), bits(b))
+
+
+def ec_case(law, nx=24, ny=6, nz=10, steps=300, **kw):
+    params = {
+        _lib.LAW_BILINEAR: dict(s0=1e-4, s0_perturb=0.01, sc=2e-3),
+        _lib.LAW_TRILINEAR: dict(s0=1e-4, s1=5e-4, sc=3e-3),
+        _lib.LAW_EXPONENTIAL: dict(s0=1e-4),
+        _lib.LAW_LINEAR: dict(sc=1e-3),
+    }[law]
+    params.update(kw)
+    return beam.build_beam("t", nx, ny, nz, 5e-3, law=law, integrator=_lib.EULER_CROMER, schedule=0,
+                           nTimeSteps=steps, appliedDisplacement=-1.5e-4, notch=(nx / 2 + 0.5, 3.5), **params)
+
+
+def compare_state(sol, run, exact=True, rtol=1e-10):
+    u, v, F, x = sol.nodal()
+    Hs, fl, fa = sol.bonds()
+    if exact:
+        assert same(u, run.u), "u"
+        assert same(v, run.v), "v"
+        assert same(x, run.x), "x"
+        assert same(F, run.F), "F"
+        assert same(Hs, run.H), "H"
+    else:
+        for name, a, b in (("u", u, run.u), ("v", v, run.v), ("F", F, run.F), ("H", Hs, run.H)):
+            scale = max(np.abs(b).max(), 1e-300)
+            err = np.abs(a - b).max() / scale
+            assert err <= rtol, (name, err)
+    assert np.array_equal(fl, run.flag), "flags"
+    assert np.array_equal(fa, run.fail), "fail"
+
+
+@pytest.mark.parametrize("law", [_lib.LAW_BILINEAR, _lib.LAW_TRILINEAR, _lib.LAW_LINEAR])
+def test_fp64_euler_cromer_bit_exact(gpu, oracle, law):
+    case = ec_case(law)
+    print(f"case: EC law {law}, {case.nNodes} nodes {case.nBonds} bonds", flush=True)
+    run = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        for blk in range(3):
+            sol.step(100)
+            run.step(100)
+            compare_state(sol, run, exact=True)
+            r, d = sol.history()
+            assert same(r, run.reactions[-100:])
+        obs = sol.observables()
+        assert obs["step"] == 300
+        assert obs["n_softening"] == int(run.flag.sum())
+        assert obs["n_failed"] == int((run.fail == 0).sum())
+        assert obs["n_broken"] == int(((run.H == 1) | (run.fail == 0)).sum())
+        if law != _lib.LAW_LINEAR:
+            assert obs["n_softening"] > 0 and obs["n_broken"] > 0
+        else:
+            assert obs["n_failed"] > 0
+
+
+def test_fp64_exponential_tolerance(gpu, oracle):
+    case = ec_case(_lib.LAW_EXPONENTIAL)
+    print(f"case: EC exponential, {case.nBonds} bonds", flush=True)
+    run = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        sol.step(300)
+        run.step(300)
+        compare_state(sol, run, exact=False, rtol=1e-10)
+        assert sol.observables()["n_softening"] > 0
+
+
+def test_fp64_adr_tolerance(gpu, oracle):
+    case = beam.build_beam("t", 24, 6, 10, 5e-3, law=_lib.LAW_BILINEAR, s0=1e-4, s0_perturb=0.01, sc=2e-3,
+                           integrator=_lib.ADR, iters_per_increment=50, increment=-1e-5)
+    print(f"case: ADR bilinear, {case.nBonds} bonds", flush=True)
+    run = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        sol.step(200)
+        run.step(200)
+        compare_state(sol, run, exact=False, rtol=1e-10)
+        obs = sol.observables()
+        Fz = (run.F[:, 2][case.bodies[0].family.astype(int) - 1] * case.cellVolume).sum()
+        assert obs["reaction_static"] == pytest.approx(Fz, rel=1e-10)
+        assert obs["cn"] > 0
+
+
+def test_full_beam_dx5_bit_exact(gpu, oracle):
+    """The 20,000-node beam: neighbour list, flags and state bit-exact after 200 EC steps."""
+    case = beam.case_bilinear_dx5(integrator=_lib.EULER_CROMER, schedule=0, nTimeSteps=400,
+                                  appliedDisplacement=-2e-4)
+    print("case: 100x10x20 EC bilinear", flush=True)
+    run = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        sol.step(200)
+        run.step(200)
+        compare_state(sol, run, exact=True)
+
+
+def test_graph_replay_equals_eager(gpu):
+    case = ec_case(_lib.LAW_BILINEAR)
+    with Solver(case, _lib.FP64) as a, Solver(case, _lib.FP64) as b:
+        a.step(120)
+        b.run_graph(40, 3)
+        b.sync()
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        assert same(a.bonds()[0], b.bonds()[0])
+        # eager work between replays does not disturb them
+        a.step(40)
+        b.run_graph(40, 1)
+        assert same(a.nodal()[0], b.nodal()[0])
+
+
+def test_node_order_does_not_matter(gpu, oracle):
+    """Renumbered nodes and a shuffled, re-oriented bond list: still bit-exact to the oracle on the
+    same (renumbered) input, i.e. the Morton slots and rows follow the given bond order."""
+    case = ec_case(_lib.LAW_BILINEAR, steps=200)
+    rng = np.random.default_rng(3)
+    nN = case.nNodes
+    perm = rng.permutation(nN)          # new id of old node n is inv[n]
+    inv = np.empty(nN, dtype=np.int64)
+    inv[perm] = np.arange(nN)
+    case.X = np.asfortranarray(case.X[perm])
+    case.DENSITY = case.DENSITY[perm]
+    BL = inv[case.BONDLIST.astype(np.int64) - 1] + 1
+    order = rng.permutation(case.nBonds)
+    BL = BL[order]
+    flip = rng.uniform(size=case.nBonds) < 0.5
+    BL[flip] = BL[flip][:, ::-1]
+    case.BONDLIST = np.asfortranarray(BL.astype(np.float64))
+    for name in ("L0", "vcf", "c", "BONDTYPE", "fail", "s0"):
+        setattr(case, name, getattr(case, name)[order])
+    for b in case.bodies:
+        b.family = np.sort(inv[b.family.astype(np.int64) - 1] + 1).astype(np.float64)
+    case.referenceNode = int(inv[case.referenceNode - 1] + 1)
+    case.cmod = tuple(int(inv[i - 1] + 1) for i in case.cmod)
+    run = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        sol.step(150)
+        run.step(150)
+        compare_state(sol, run, exact=True)
+
+
+def test_setup_rejects_inconsistent_inputs(gpu):
+    case = ec_case(_lib.LAW_BILINEAR)
+    bad = case.L0.copy()
+    bad[5] *= 1 + 1e-15
+    case.L0 = bad
+    with pytest.raises(_lib.BBPDError, match="UNDEFORMEDLENGTH"):
+        Solver(case, _lib.FP64)
+
+
+def test_mixed_precision_error_report(gpu):
+    """FP32-mixed against plain FP64 on the same run: every result within its stated bound."""
+    case = ec_case(_lib.LAW_BILINEAR, steps=400)
+    with Solver(case, _lib.FP64) as a, Solver(case, _lib.MIXED) as b:
+        a.step(400)
+        b.step(400)
+        ua, va, Fa, xa = a.nodal()
+        ub, vb, Fb, xb = b.nodal()
+        Ha, fla, _ = a.bonds()
+        Hb, flb, _ = b.bonds()
+        ra, _ = a.history()
+        rb, _ = b.history()
+        err_u = np.abs(ua - ub).max() / np.abs(ua).max()
+        err_F = np.abs(Fa - Fb).max() / np.abs(Fa).max()
+        flips = int((fla != flb).sum())
+        broken_a, broken_b = int((Ha == 1).sum()), int((Hb == 1).sum())
+        print(f"mixed vs fp64: u {err_u:.2e} F {err_F:.2e} flag flips {flips} broken {broken_a} vs {broken_b}")
+        assert err_u < 2e-3
+        assert flips <= max(5, 0.02 * fla.sum())
+        assert abs(broken_a - broken_b) <= max(5, 0.02 * broken_a)

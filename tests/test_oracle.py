@@ -1,0 +1,339 @@
+"""Pins of the oracle (oracle/bbpd_oracle.c) by closed forms, invariants, brute force and real OpenMP.
+
+The reference C needs MATLAB's mex.h and cannot be built here, so each oracle routine is checked
+against what the mathematics fixes, chosen so that a dropped term, a wrong sign or index, or a wrong
+summation order fails at least one test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from bb_pd_amd import beam
+
+
+def z(n):
+    return np.zeros(n)
+
+
+def z3(n):
+    return np.zeros((n, 3), order="F")
+
+
+# ---------------------------------------------------------------- deformed length / stretch
+def test_deformedlength_hand_values(oracle):
+    X = np.asfortranarray([[0.0, 0, 0], [1, 0, 0], [0, 4, 0], [3, 0, 0]])
+    x = np.asfortranarray([[0.0, 0, 0], [1.25, 0, 0], [0, 4, 0], [3, 4, 0]])
+    BL = np.asfortranarray([[1.0, 2], [1, 3], [3, 4]])
+    L0 = np.array([1.0, 4.0, 5.0])  # |X2-X1|, |X3-X1|, |X4-X3|
+    l, dX, dY, dZ, s = z(3), z(3), z(3), z(3), z(3)
+    oracle.calculatedeformedlength(x, BL, L0, l, dX, dY, dZ, s)
+    assert list(dX) == [1.25, 0.0, 3.0]
+    assert list(dY) == [0.0, 4.0, 0.0]
+    assert list(l) == [1.25, 4.0, 3.0]          # bond 3: (3,4,0)-(0,4,0) -> length 3
+    assert list(s) == [0.25, 0.0, (3.0 - 5.0) / 5.0]
+    # column 2 of BONDLIST is the j end: dX = x_j - x_i (sign pin)
+    BLr = np.asfortranarray(BL[:, ::-1])
+    oracle.calculatedeformedlength(x, BLr, L0, l, dX, dY, dZ, s)
+    assert list(dX) == [-1.25, -0.0, -3.0]
+
+
+def test_stretch_rigid_translation_and_dilation(oracle):
+    case = beam.build_beam("t", 6, 5, 4, 1.0, law=1, sc=1.0)
+    X = case.X
+    x = np.asfortranarray(X + np.array([3.0, -2.0, 7.0]))       # integer translation: exact
+    nB = case.nBonds
+    l, dX, dY, dZ, s = z(nB), z(nB), z(nB), z(nB), z(nB)
+    oracle.calculatedeformedlength(x, case.BONDLIST, case.L0, l, dX, dY, dZ, s)
+    assert np.all(s == 0.0)
+    e = 1.0 / 16
+    x = np.asfortranarray(X * (1 + e))
+    oracle.calculatedeformedlength(x, case.BONDLIST, case.L0, l, dX, dY, dZ, s)
+    assert np.max(np.abs(s - e)) <= 4 * np.finfo(float).eps
+
+
+# ---------------------------------------------------------------- bond force
+def test_bondforce_hand_value_and_nan(oracle):
+    # p = fail*c*(1-H)*(s-sp)*V*vcf = 1*2*0.5*(0.375-0.125)*4*1 = 1 ; f = p*(dX/l)
+    fail, c, H, s, sp, vcf = [np.array([v, v]) for v in (1.0, 2.0, 0.5, 0.375, 0.125, 1.0)]
+    dX, dY, dZ, l = np.array([3.0, 0.0]), np.array([4.0, 0.0]), np.array([0.0, 0.0]), np.array([5.0, 0.0])
+    fX, fY, fZ = z(2), z(2), z(2)
+    oracle.calculatebondforces(fX, fY, fZ, fail, dX, dY, dZ, l, s, sp, c, 4.0, vcf, H)
+    assert fX[0] == 0.6 and fY[0] == 0.8 and fZ[0] == 0.0
+    assert fX[1] == 0.0 and fY[1] == 0.0 and fZ[1] == 0.0   # l = 0 -> 0/0 -> NaN -> 0
+    # a failed bond carries nothing; vcf scales linearly
+    fail2 = np.array([0.0, 1.0])
+    vcf2 = np.array([0.5, 0.5])
+    oracle.calculatebondforces(fX, fY, fZ, fail2, dX, dY, dZ, l, s, sp, c, 4.0, vcf2, H)
+    assert fX[0] == 0.0
+
+
+def _random_bonds(rng, nN, nB):
+    i = rng.integers(1, nN + 1, nB)
+    j = rng.integers(1, nN + 1, nB)
+    j = np.where(j == i, (i % nN) + 1, j)
+    return np.asfortranarray(np.stack([i, j], 1).astype(float))
+
+
+# ---------------------------------------------------------------- nodal reduction
+def test_nodal_serial_matches_exact_sums(oracle):
+    rng = np.random.default_rng(1)
+    nN, nB = 50, 400
+    BL = _random_bonds(rng, nN, nB)
+    f = [rng.standard_normal(nB) for _ in range(3)]
+    F = z3(nN)
+    oracle.calculatenodalforces_serial(BL, F, *f)
+    for d in range(3):
+        exact = np.zeros(nN)
+        for n in range(nN):
+            terms = [f[d][k] for k in range(nB) if BL[k, 0] == n + 1] + \
+                    [-f[d][k] for k in range(nB) if BL[k, 1] == n + 1]
+            exact[n] = math.fsum(terms)
+        np.testing.assert_allclose(F[:, d], exact, rtol=0, atol=1e-12)
+        assert abs(math.fsum(F[:, d])) < 1e-12   # Newton's third law
+
+
+def test_nodal_thread_order_equals_real_openmp(oracle):
+    """The sequential T-thread emulation reproduces a real OpenMP team's result bit for bit, and
+    differs from the serial order on data with a wide dynamic range (so the check has teeth)."""
+    rng = np.random.default_rng(7)
+    nN, nB = 40, 997
+    BL = _random_bonds(rng, nN, nB)
+    f = [rng.standard_normal(nB) * 10.0 ** rng.integers(-8, 9, nB) for _ in range(3)]
+    Fs = z3(nN)
+    oracle.calculatenodalforces_serial(BL, Fs, *f)
+    differs = False
+    for T in (1, 2, 3, 8):
+        Fe, Fr = z3(nN), z3(nN)
+        sc_e = [z(nN * T) for _ in range(3)]
+        sc_r = [z(nN * T) for _ in range(3)]
+        oracle.calculatenodalforces_openmp(BL, Fe, *f, *sc_e, T)
+        oracle.calculatenodalforces_omp_real(BL, Fr, *f, *sc_r, T)
+        assert np.array_equal(Fe.view(np.int64), Fr.view(np.int64)), T
+        for a, b in zip(sc_e, sc_r):
+            assert np.array_equal(a, b)
+        if T == 1:
+            assert np.array_equal(Fe, Fs)
+        elif not np.array_equal(Fe, Fs):
+            differs = True
+    assert differs
+
+
+# ---------------------------------------------------------------- Euler-Cromer
+def test_eulercromer_single_dof(oracle):
+    F = np.asfortranarray([[2.0, 2.0, 2.0]])
+    v = np.asfortranarray([[1.0, 1.0, 1.0]])
+    u = z3(1)
+    X = np.asfortranarray([[1.0, 2.0, 3.0]])
+    x = X.copy(order="F")
+    a = z3(1)
+    cons = np.asfortranarray([[0.0, 1.0, 0.0]])
+    oracle.timeintegrationeulercromer(a, F, 2.0, v, np.array([4.0]), cons, 0.5, u, x, X)
+    # a = (2 - 2*1)/4 = 0 with damping 2
+    assert list(a[0]) == [0.0, 0.0, 0.0]
+    F = np.asfortranarray([[2.0, 2.0, 6.0]])
+    v = np.asfortranarray([[1.0, 1.0, 1.0]])
+    u = z3(1)
+    oracle.timeintegrationeulercromer(a, F, 0.0, v, np.array([4.0]), cons, 0.5, u, x, X)
+    assert list(a[0]) == [0.5, 0.0, 1.5]                 # DOF 2 constrained
+    assert list(v[0]) == [1.25, 1.0, 1.75]               # v += a DT
+    assert list(u[0]) == [0.625, 0.5, 0.875]             # u += v(new) DT (Euler-Cromer, not Euler)
+    assert list(x[0]) == [1.625, 2.5, 3.875]
+
+
+# ---------------------------------------------------------------- softening laws
+def test_bilinear_closed_forms_and_quirk(oracle):
+    s0, sc = 1e-4, 2e-3
+    s = np.array([0.5e-4, s0, 3e-4, 1e-3, 1.5e-3, sc, 3e-3])
+    n = s.size
+    H, flag, BT = z(n), z(n), z(n)
+    oracle.calculatebondsofteningfactor(s, s0, sc, flag, H, BT)
+    assert list(flag) == [0, 0, 1, 1, 1, 1, 1]
+    assert H[0] == 0 and H[1] == 0
+    force = (1 - H) * s
+    lin = s0 * (sc - s) / (sc - s0)                   # linear descent from s0 at s0 to 0 at sc
+    np.testing.assert_allclose(force[2:6], lin[2:6], rtol=1e-12, atol=1e-18)
+    assert abs(H[5] - 1) < 1e-15 and H[6] == 1.0
+    # irreversibility: unloading keeps H; compression of a flagged bond breaks it (the .m quirk)
+    H_before = H.copy()
+    s2 = np.array([0.5e-4, s0, 1e-4, 1e-4, 1e-4, 1e-4, 1e-4])
+    oracle.calculatebondsofteningfactor(s2, s0, sc, flag, H, BT)
+    assert np.array_equal(H, H_before)
+    s3 = np.full(n, -1e-5)
+    oracle.calculatebondsofteningfactor(s3, s0, sc, flag, H, BT)
+    assert list(H) == [0, 0, 1, 1, 1, 1, 1]
+    # non-concrete bonds are never flagged; per-bond s0 arrays broadcast like scalars
+    H, flag, BT = z(n), z(n), np.full(n, 2.0)
+    oracle.calculatebondsofteningfactor(s, np.full(n, s0), sc, flag, H, BT)
+    assert not flag.any() and not H.any()
+
+
+def test_trilinear_closed_forms(oracle):
+    s0, s1, sc, beta = 1e-4, 5e-4, 3e-3, 0.25
+    eps = 1e-12
+    s = np.array([s0 + eps, s1, s1 + eps, 0.5 * (s1 + sc), sc, sc + 1e-6])
+    n = s.size
+    H, flag, BT = z(n), z(n), z(n)
+    oracle.calculatebsftrilinear(s, s0, s1, sc, H, BT, flag)
+    force = (1 - H) * s
+    assert force[0] == pytest.approx(s0, rel=1e-7)             # peak s0 at s0
+    assert force[1] == pytest.approx(beta * s0, rel=1e-12)     # knee beta*s0 at s1 (branch 1)
+    assert force[2] == pytest.approx(beta * s0, rel=1e-7)      # continuous into branch 2
+    assert force[3] == pytest.approx(0.5 * beta * s0, rel=1e-12)  # linear to 0 at sc
+    assert abs(force[4]) < 1e-18 and H[5] == 1.0
+    assert list(flag) == [1, 1, 0, 0, 0, 0]                    # flag only in branch 1 (.m:87)
+
+
+def test_exponential_closed_forms_and_fracture_energy(oracle):
+    dx = 1.25e-3
+    delta = 3.015 * dx
+    c0 = 12 * 37e9 / (math.pi * delta ** 4)
+    s0, GF, k, alpha = 1e-4, 125.2, 25.0, 0.25
+    sc = float(beam.exponential_sc(np.array([c0]), s0, GF, delta, k, alpha)[0])
+    s = np.linspace(s0, sc, 400001)
+    s[0] = s0 * (1 + 1e-12)
+    n = s.size
+    bsf, H, BT, flag = z(n), z(n), z(n), z(n)
+    oracle.calculatebsfexponential(s, np.full(n, s0), np.full(n, sc), k, alpha, bsf, H, BT, flag)
+    assert H[0] < 1e-9 and abs(H[-1] - 1) < 1e-14
+    assert np.all(np.diff(H) >= -1e-15)                        # softening factor is monotone
+    # the sc formula (Gregoire_beam_tests.m:235) makes the softening-branch energy equal GF:
+    # GF = (pi delta^5 / 5) c * integral_{s0}^{sc} (1 - bsf(s)) s ds
+    g = (1 - H) * s
+    G = math.pi * delta ** 5 / 5 * c0 * np.trapezoid(g, s)
+    assert G == pytest.approx(GF, rel=1e-6)
+    # per-step scratch: a bond beyond sc is broken in one step even from below s0 (C semantics)
+    s1 = np.array([0.5 * s0, 2 * sc])
+    bsf, H, flag = z(2), z(2), z(2)
+    oracle.calculatebsfexponential(s1, np.full(2, s0), np.full(2, sc), k, alpha, bsf, H, z(2), flag)
+    assert list(H) == [0.0, 1.0] and list(flag) == [0.0, 0.0]
+
+
+def test_brittle_latch(oracle):
+    fail = np.ones(4)
+    BT = np.array([0.0, 1.0, 2.0, 0.0])
+    s = np.array([2e-3, 2e-3, 2e-3, 0.5e-3])
+    oracle.calculatebondfailure(fail, 0.0, BT, s, 1e-3, 5e-3)
+    assert list(fail) == [0, 1, 1, 1]          # interface needs 3 sc, steel needs sc_steel
+    s = np.array([0.0, 4e-3, 6e-3, 0.0])
+    oracle.calculatebondfailure(fail, 0.0, BT, s, 1e-3, 5e-3)
+    assert list(fail) == [0, 0, 0, 1]          # irreversible: bond 1 stays failed at s = 0
+
+
+# ---------------------------------------------------------------- ADR
+def test_adr_damping_rayleigh_quotient(oracle):
+    # one free DOF of a linear spring: F = -k u.  With v_prev = (u - u_prev)/DT the quotient gives
+    # cn = 2 sqrt(k/m); DOFs with zero velocity are skipped.
+    kk, m, DT = 0.25, 1.0, 1.0
+    u_prev, u = 0.5, 0.75
+    F = np.asfortranarray([[-kk * u, 123.0, 0.0]])
+    Fp = np.asfortranarray([[-kk * u_prev, -7.0, 0.0]])
+    vp = np.asfortranarray([[(u - u_prev) / DT, 0.0, 0.0]])
+    uu = np.asfortranarray([[u, 0.0, 0.0]])
+    cn = oracle.calculatedampingcoefficient(F, np.array([m]), Fp, DT, vp, uu)
+    assert cn == pytest.approx(2 * math.sqrt(kk / m), rel=1e-14)
+    F[0, 0], Fp[0, 0] = -4 * u, -4 * u_prev
+    assert oracle.calculatedampingcoefficient(F, np.array([m]), Fp, DT, vp, uu) == 1.9   # cn > 2 -> 1.9
+    assert oracle.calculatedampingcoefficient(F, np.array([m]), Fp, DT, vp, z3(1)) == 0.0
+
+
+def test_adr_update_recurrence(oracle):
+    DT, m = 1.0, np.array([4.0])
+    F = np.asfortranarray([[8.0, -4.0, 0.0]])
+    v = z3(1)
+    u = z3(1)
+    cons = np.asfortranarray([[0.0, 0.0, 1.0]])
+    oracle.timeintegrationADR(1, DT, v, F, m, 0.0, v, u, cons)
+    assert list(v[0]) == [1.0, -0.5, 0.0]          # v = DT F / (2m)
+    assert list(u[0]) == [1.0, -0.5, 0.0]
+    cn = 0.5
+    oracle.timeintegrationADR(2, DT, v, F, m, cn, v, u, cons)
+    # v2 = ((2 - cn DT) v1 + 2 DT F/m) / (2 + cn DT) = (1.5*1 + 4)/2.5 = 2.2
+    assert v[0, 0] == pytest.approx(2.2, rel=1e-15) and u[0, 0] == pytest.approx(3.2, rel=1e-15)
+    assert v[0, 2] == 0.0 and u[0, 2] == 0.0
+
+
+def test_mass_vector(oracle):
+    m = oracle.buildmassmatrix(0.5, 8.0, 2.0, np.array([10.0]))
+    assert m[0] == 0.25 * 4.0 * (10 * 0.25) * (8.0 / 0.5)
+
+
+# ---------------------------------------------------------------- contact
+def test_contact_projection_and_reaction(oracle):
+    X = np.asfortranarray([[0.0, 0.0, 0.0], [0.3, 0.0, 0.0], [5.0, 0.0, 0.0]])
+    x = X.copy(order="F")
+    u, v = z3(3), z3(3)
+    rho = np.array([2.0, 2.0, 2.0])
+    fam = np.array([1.0, 2.0, 3.0])
+    R, cz0, DT, V = 1.0, 0.75, 0.5, 0.125           # node 1 sits 0.25 inside a circle centred above
+    fz = oracle.calculatecontactforce(fam, 0.0, cz0, R, 0.0, X, x, u, v, DT, V, rho)
+    assert x[0, 0] == 0.0 and x[0, 2] == -0.25       # projected straight down to the circle
+    assert u[0, 2] == -0.25 and v[0, 2] == -0.5      # v = (u - u_old)/DT
+    assert u[0, 1] == 0.0 and v[0, 1] == 0.0         # no motion across the width
+    d = math.hypot(0.3, 0.75)
+    assert x[1, 0] == pytest.approx(0.3 / d, rel=1e-15)
+    assert x[2, 0] == 5.0                            # outside: untouched
+    dvz = [v[0, 2], v[1, 2]]
+    expected = sum(-1 * 2.0 * dv / DT * V for dv in dvz)   # reaction = momentum change
+    assert fz == pytest.approx(expected, rel=1e-15)
+    # the centre move is not cumulative: the same displacement again moves nothing further
+    x2, u2, v2 = x.copy(order="F"), u.copy(order="F"), v.copy(order="F")
+    oracle.calculatecontactforce(fam, 0.0, cz0, R, -0.1, X, x, u, v, DT, V, rho)
+    oracle.calculatecontactforce(fam, 0.0, cz0, R, -0.1, X, x2, u2, v2, DT, V, rho)
+    oracle.calculatecontactforce(fam, 0.0, cz0, R, -0.1, X, x2, u2, v2, DT, V, rho)
+    assert x2[0, 2] == x[0, 2] == pytest.approx(-0.35, rel=1e-15)
+
+
+def test_smoothstep(oracle):
+    N, a = 1001, -1e-3
+    assert oracle.smoothstepdata(1, 1, N, 0, a) == 0.0
+    assert oracle.smoothstepdata(N, 1, N, 0, a) == a
+    assert oracle.smoothstepdata(501, 1, N, 0, a) == a / 2
+    d1 = oracle.smoothstepdata(2, 1, N, 0, a)
+    assert abs(d1) < 1e-8 * abs(a)                      # zero slope at the start
+    assert abs(a - oracle.smoothstepdata(N - 1, 1, N, 0, a)) < 1e-8 * abs(a)
+
+
+def test_damage_field_and_energy(oracle):
+    BL = np.asfortranarray([[1.0, 2], [1, 3], [2, 3]])
+    unbroken = np.array([0.0, 1.0, 0.0])
+    d = oracle.calculatedamage(BL, unbroken, np.array([2.0, 2.0, 2.0]))
+    assert list(d) == [0.5, 1.0, 0.5]
+    # damage energy at s >= sc equals c L0 V^2 times the softening-branch integral of (1-bsf)s
+    s0, sc, k, alpha = 1e-4, 2e-3, 25.0, 0.25
+    ss = np.linspace(s0, sc, 200001)
+    n = ss.size
+    bsf, H = z(n), z(n)
+    oracle.calculatebsfexponential(ss, np.full(n, s0), np.full(n, sc), k, alpha, bsf, H, z(n), z(n))
+    integral = np.trapezoid((1 - H) * ss, ss)
+    hist = z(2)
+    c, L0, V = np.array([3.0, 3.0]), np.array([2.0, 2.0]), 0.5
+    oracle.calculatedamageenergy(c, np.full(2, s0), np.full(2, sc), L0, np.array([3e-3, 0.5e-4]), k, alpha, V, hist)
+    assert hist[0] == pytest.approx(3.0 * 2.0 * 0.25 * integral, rel=1e-7)
+    assert hist[1] == 0.0
+
+
+# ---------------------------------------------------------------- the whole step
+def test_step_order_single_bond(oracle):
+    """Two nodes, one bond, Euler-Cromer: one full oracle step equals the hand sequence."""
+    class Case:
+        pass
+    c = Case()
+    c.X = np.asfortranarray([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    c.BONDLIST = np.asfortranarray([[1.0, 2.0]])
+    c.L0, c.vcf, c.c, c.BONDTYPE, c.fail = [np.array([v]) for v in (1.0, 1.0, 2.0, 0.0, 1.0)]
+    c.s0, c.sc, c.s1, c.k, c.alpha, c.sc_steel, c.failureFunctionality = 0.5, 1.0, 0.0, 25.0, 0.25, 1.0, 0.0
+    c.DENSITY = np.array([1.0, 1.0])
+    c.CONSTRAINTFLAG = np.asfortranarray([[1.0, 1.0, 1.0], [0.0, 0.0, 0.0]])
+    c.mass = None
+    c.cellVolume, c.DT, c.DAMPING = 1.0, 0.5, 0.0
+    c.law, c.integrator = 1, 0
+    c.bodies = []
+    c.displacement = lambda i: 0.0
+    run = oracle.OracleRun(c)
+    run.x[1, 0] = 1.25                                  # pre-stretched: s = 0.25 (< s0, elastic)
+    run.step(1)
+    # f = 2 * 0.25 * 1 = 0.5 toward node 1 on node 2; node 1 is fixed
+    assert run.F[0, 0] == 0.5 and run.F[1, 0] == -0.5
+    assert run.v[1, 0] == -0.25 and run.u[1, 0] == -0.125 and run.x[1, 0] == 0.875
+    assert run.u[0, 0] == 0.0
