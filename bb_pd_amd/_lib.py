@@ -84,6 +84,8 @@ SIGNATURES = {
     "bbpd_solver_observables": (_I, [_V, C.POINTER(Observables)]),
     "bbpd_solver_get_nodal": (_I, [_V, _P, _P, _P, _P]),
     "bbpd_solver_get_bonds": (_I, [_V, _P, _P, _P]),
+    "bbpd_solver_checkpoint": (_I, [_V, C.POINTER(_L), _P, _P, _P, _P, _P, _P, _P, _P]),
+    "bbpd_solver_restore": (_I, [_V, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "bbpd_solver_history": (_I, [_V, _P, _P, _I, C.POINTER(_I)]),
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_set_timing": (_I, [_V, _I]),
@@ -103,6 +105,14 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise BBPDError(f"{LIB_PATH} is missing: build it with bb_pd_amd.build.build_library()")
+        # One HIP runtime per process: torch ships its own libamdhip64 / libhsa-runtime64 and its
+        # libraries ask for them by unversioned name, so they would not reuse an already-loaded
+        # /opt/rocm copy and a second runtime would find no device.  Loading torch first registers
+        # its runtime under the versioned sonames, which libbbpd_hip then binds to.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -24,6 +24,7 @@
 //     runs in the same kernel that gathers its neighbours.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -42,11 +43,11 @@ constexpr uint32_t TYPE_SHIFT = 28;
 constexpr uint32_t FAILED_BIT = 1u << 30;
 constexpr uint32_t FLAG_BIT = 1u << 31;
 constexpr int HIST_CAP = 1 << 20;
-constexpr int CONTACT_TILE = 2048;
 constexpr int N_HIST = 4; // reaction, static reaction, deflection, CMOD
 
 struct DevScalars {
     long long step; // steps completed
+    unsigned int done; // blocks of the step-end kernel finished this step
     double cn;
     double cn1, cn2;
     double reaction[BBPD_MAX_BODIES];
@@ -180,21 +181,47 @@ __global__ __launch_bounds__(BLOCK) void k_bond_fp64(Params P)
     const double xn0 = x0[n], xn1 = x1[n], xn2 = x2[n];
     const double Xn0 = X0[n], Xn1 = X1[n], Xn2 = X2[n];
     long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
+    // software pipeline: entry kk+1's stream words and neighbour gathers are in flight while
+    // entry kk is computed (loads only move; the arithmetic and its order are unchanged)
+    uint32_t wN = 0;
+    double HN = 0, cN = 0, s0N = 0, scN = 0, xm0N = 0, xm1N = 0, xm2N = 0, Xm0N = 0, Xm1N = 0, Xm2N = 0;
+    if (len > 0) {
+        wN = P.nbr[e];
+        HN = hist[e];
+        cN = cc[e];
+        if (S0A) s0N = s0e[e];
+        if (SCA) scN = sce[e];
+        const int mN = (int)(wN & NODE_MASK);
+        xm0N = x0[mN], xm1N = x1[mN], xm2N = x2[mN];
+        Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
+    }
     for (int kk = 0; kk < len; kk++, e += SLICE) {
-        uint32_t w = P.nbr[e];
+        uint32_t w = wN;
         const uint32_t w0 = w;
-        const int m = (int)(w & NODE_MASK);
+        double H = HN;
+        const double ccur = cN, s0c = s0N, scc = scN;
+        const double xm0 = xm0N, xm1 = xm1N, xm2 = xm2N, Xm0 = Xm0N, Xm1 = Xm1N, Xm2 = Xm2N;
+        if (kk + 1 < len) {
+            const long long e1 = e + SLICE;
+            wN = P.nbr[e1];
+            HN = hist[e1];
+            cN = cc[e1];
+            if (S0A) s0N = s0e[e1];
+            if (SCA) scN = sce[e1];
+            const int mN = (int)(wN & NODE_MASK);
+            xm0N = x0[mN], xm1N = x1[mN], xm2N = x2[mN];
+            Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
+        }
         // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n): exact negation of the
         // reference's x_j - x_i at the j end, so |.|, s and the force magnitude are identical
-        const double dx = x0[m] - xn0, dy = x1[m] - xn1, dz = x2[m] - xn2;
+        const double dx = xm0 - xn0, dy = xm1 - xn1, dz = xm2 - xn2;
         const double l = sqrt(dx * dx + dy * dy + dz * dz);
-        const double ex = Xn0 - X0[m], ey = Xn1 - X1[m], ez = Xn2 - X2[m];
+        const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
         const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
         const double s = (l - L0) / L0;
-        double H = hist[e];
         const double H0 = H;
-        const double s0 = S0A ? s0e[e] : P.s0;
-        const double sc = SCA ? sce[e] : P.sc;
+        const double s0 = S0A ? s0c : P.s0;
+        const double sc = SCA ? scc : P.sc;
         apply_law<LAW, double>(s, s0, sc, P.s1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
         if (w != w0) P.nbr[e] = w;
         if (__double_as_longlong(H) != __double_as_longlong(H0)) hist[e] = H;
@@ -205,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_fp64(Params P)
         else vcf = 0;
         // calculatebondforcesMex.c:34-40 with stretchPlastic = 0 (s - 0 == s exactly)
         const double fail = (w & FAILED_BIT) ? 0.0 : 1.0;
-        const double p = fail * cc[e] * (1 - H) * s * P.V * vcf;
+        const double p = fail * ccur * (1 - H) * s * P.V * vcf;
         double gx = p * (dx / l), gy = p * (dy / l), gz = p * (dz / l);
         if (gx != gx) gx = 0.0;
         if (gy != gy) gy = 0.0;
@@ -295,26 +322,52 @@ __global__ __launch_bounds__(BLOCK) void k_bond_mixed(Params P)
     const double un0 = u0[n], un1 = u1[n], un2 = u2[n];
     const double Xn0 = X0[n], Xn1 = X1[n], Xn2 = X2[n];
     long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
+    // software pipeline as in the FP64 kernel: entry kk+1 is loaded while entry kk is computed
+    uint32_t wN = 0;
+    float HN = 0.f, kN = 0.f, s0N = 0.f, scN = 0.f;
+    double um0N = 0, um1N = 0, um2N = 0, Xm0N = 0, Xm1N = 0, Xm2N = 0;
+    if (len > 0) {
+        wN = P.nbr[e];
+        HN = hist[e];
+        kN = kap[e];
+        if (S0A) s0N = s0e[e];
+        if (SCA) scN = sce[e];
+        const int mN = (int)(wN & NODE_MASK);
+        um0N = u0[mN], um1N = u1[mN], um2N = u2[mN];
+        Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
+    }
     for (int kk = 0; kk < len; kk++, e += SLICE) {
-        uint32_t w = P.nbr[e];
+        uint32_t w = wN;
         const uint32_t w0 = w;
-        const int m = (int)(w & NODE_MASK);
-        const float xi0 = (float)(X0[m] - Xn0), xi1 = (float)(X1[m] - Xn1), xi2 = (float)(X2[m] - Xn2);
-        const float et0 = (float)(u0[m] - un0), et1 = (float)(u1[m] - un1), et2 = (float)(u2[m] - un2);
+        float H = HN;
+        const float kcur = kN, s0cur = s0N, sccur = scN;
+        const double um0 = um0N, um1 = um1N, um2 = um2N, Xm0 = Xm0N, Xm1 = Xm1N, Xm2 = Xm2N;
+        if (kk + 1 < len) {
+            const long long e1 = e + SLICE;
+            wN = P.nbr[e1];
+            HN = hist[e1];
+            kN = kap[e1];
+            if (S0A) s0N = s0e[e1];
+            if (SCA) scN = sce[e1];
+            const int mN = (int)(wN & NODE_MASK);
+            um0N = u0[mN], um1N = u1[mN], um2N = u2[mN];
+            Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
+        }
+        const float xi0 = (float)(Xm0 - Xn0), xi1 = (float)(Xm1 - Xn1), xi2 = (float)(Xm2 - Xn2);
+        const float et0 = (float)(um0 - un0), et1 = (float)(um1 - un1), et2 = (float)(um2 - un2);
         const float y0 = xi0 + et0, y1 = xi1 + et1, y2 = xi2 + et2;
         const float L0 = sqrtf(xi0 * xi0 + xi1 * xi1 + xi2 * xi2);
         const float l = sqrtf(y0 * y0 + y1 * y1 + y2 * y2);
         const float num = 2.0f * (xi0 * et0 + xi1 * et1 + xi2 * et2) + (et0 * et0 + et1 * et1 + et2 * et2);
         const float s = num / (L0 * (l + L0));
-        float H = hist[e];
         const float H0 = H;
-        const float s0 = S0A ? s0e[e] : s0c;
-        const float sc = SCA ? sce[e] : scc;
+        const float s0 = S0A ? s0cur : s0c;
+        const float sc = SCA ? sccur : scc;
         apply_law<LAW, float>(s, s0, sc, s1c, kc, ac, dc, sstc, ffc, H, w);
         if (w != w0) P.nbr[e] = w;
         if (__float_as_uint(H) != __float_as_uint(H0)) hist[e] = H;
         const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
-        const float p = fail * kap[e] * (1.0f - H) * s / l;
+        const float p = fail * kcur * (1.0f - H) * s / l;
         float gx = p * y0, gy = p * y1, gz = p * y2;
         if (gx != gx) gx = 0.0f;
         if (gy != gy) gy = 0.0f;
@@ -465,34 +518,39 @@ __device__ inline double schedule_disp(const StepEnd &S, long long i)
 template <bool MIXED>
 __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
 {
-    __shared__ double dfz[CONTACT_TILE];
-    __shared__ unsigned char hit[CONTACT_TILE];
+    // one block per contact body (families are disjoint); the last block to finish records the
+    // step's observables and advances the step counter
+    __shared__ double vals[1024];
+    __shared__ int wofs[17];
+    __shared__ int last_block;
     const int nxt = (int)((P.ds->step + 1) & 1);
     const long long it = P.ds->step + 1;
     double *pos[3] = {P.p0[nxt], P.p1[nxt], P.p2[nxt]};
     double *uu[3] = {P.u0, P.u1, P.u2};
     double *vv[3] = {P.v0, P.v1, P.v2};
     const double *XX[3] = {P.X0, P.X1, P.X2};
-    const double disp = schedule_disp(S, it);
-    double reac0 = 0;
-    for (int bdy = 0; bdy < S.nBodies; bdy++) {
+    const int bdy = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    double total = 0, rs = 0;
+    if (bdy < S.nBodies) {
+        const double disp = schedule_disp(S, it);
         const double cx = S.cx[bdy], czm = S.cz[bdy] + (S.moves[bdy] ? disp : 0.0), R = S.R[bdy];
-        double total = 0;
-        for (int base = 0; base < S.nFam[bdy]; base += CONTACT_TILE) {
-            const int cnt = min(CONTACT_TILE, S.nFam[bdy] - base);
-            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-                const int i = S.fam[bdy][base + t];
+        const int nf = S.nFam[bdy];
+        for (int base = 0; base < nf; base += blockDim.x) {
+            const int t = base + threadIdx.x;
+            bool hit = false;
+            double d_z = 0;
+            if (t < nf) {
+                const int i = S.fam[bdy][t];
                 double xc[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) xc[d] = MIXED ? XX[d][i] + pos[d][i] : pos[d][i];
                 const double ddx = xc[0] - cx, ddz = xc[2] - czm;
                 const double dist = sqrt((ddx * ddx) + (ddz * ddz));
-                hit[t] = 0;
                 if (dist < R) {
-                    hit[t] = 1;
+                    hit = true;
                     xc[0] = cx + (ddx / dist) * R;
                     xc[2] = czm + (ddz / dist) * R;
-                    double d_z = 0;
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const double uo = MIXED ? pos[d][i] : uu[d][i];
@@ -508,36 +566,53 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
                         vv[d][i] = nv;
                         if (d == 2) d_z = -1 * P.rho[i] * (nv - vo) / P.DT * P.V;
                     }
-                    dfz[t] = d_z;
                 }
             }
+            // compact the members in contact, in family order, then sum them in that order
+            // (the order of MATLAB's sum over dpenfz; members not in contact add nothing)
+            const unsigned long long mask = __ballot(hit);
+            if (lane == 0) wofs[wv + 1] = __popcll(mask);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                wofs[0] = 0;
+                for (int q = 1; q <= nw; q++) wofs[q] += wofs[q - 1];
+            }
+            __syncthreads();
+            if (hit) vals[wofs[wv] + __popcll(mask & ((1ull << lane) - 1))] = d_z;
             __syncthreads();
             if (threadIdx.x == 0)
-                for (int t = 0; t < cnt; t++)
-                    if (hit[t]) total = total + dfz[t];
+                for (int q = 0; q < wofs[nw]; q++) total = total + vals[q];
             __syncthreads();
         }
-        if (threadIdx.x == 0) {
-            P.ds->reaction[bdy] = total;
-            if (bdy == 0) reac0 = total;
+        if (bdy == 0) {
+            // calculatereactionforceFast.m:3-5 over body 0's family: F_z * cellVolume (fixed tree)
+            double part = 0;
+            for (int t = threadIdx.x; t < nf; t += blockDim.x) part = part + P.F2[S.fam[0][t]] * P.V;
+            rs = bbpd::block_sum_fixed<1024>(part, vals);
         }
     }
     if (threadIdx.x == 0) {
-        // calculatereactionforceFast.m:3-5 over body 0's family (F_z * cellVolume, family order)
-        double rs = 0;
-        if (S.nBodies > 0)
-            for (int t = 0; t < S.nFam[0]; t++) rs = rs + P.F2[S.fam[0][t]] * P.V;
-        P.ds->reaction_static = rs;
-        const double *uz = MIXED ? pos[2] : uu[2];
-        const double *ux = MIXED ? pos[0] : uu[0];
-        double defl = S.ref_slot >= 0 ? uz[S.ref_slot] : 0.0;
-        double cmod = (S.cmod1 >= 0 && S.cmod2 >= 0) ? ux[S.cmod2] - ux[S.cmod1] : 0.0;
-        double *h = S.hist + (size_t)(P.ds->step % HIST_CAP) * N_HIST;
-        h[0] = reac0;
-        h[1] = rs;
+        if (bdy < S.nBodies) P.ds->reaction[bdy] = total;
+        if (bdy == 0) P.ds->reaction_static = rs;
+        __threadfence();
+        const unsigned ticket = atomicAdd(&P.ds->done, 1u);
+        last_block = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last_block && threadIdx.x == 0) {
+        __threadfence();
+        volatile DevScalars *vds = P.ds;
+        const volatile double *uz = MIXED ? pos[2] : uu[2];
+        const volatile double *ux = MIXED ? pos[0] : uu[0];
+        const double defl = S.ref_slot >= 0 ? uz[S.ref_slot] : 0.0;
+        const double cmod = (S.cmod1 >= 0 && S.cmod2 >= 0) ? ux[S.cmod2] - ux[S.cmod1] : 0.0;
+        double *h = S.hist + (size_t)(it - 1) % HIST_CAP * N_HIST;
+        h[0] = S.nBodies > 0 ? vds->reaction[0] : 0.0;
+        h[1] = vds->reaction_static;
         h[2] = defl;
         h[3] = cmod;
-        P.ds->step = it;
+        vds->done = 0u;
+        vds->step = it;
     }
 }
 
@@ -700,8 +775,9 @@ int launch_step(bbpd_solver *s, bool allow_timing)
         else hipLaunchKernelGGL(k_adr_update<false>, dim3(g2), dim3(BLOCK), 0, s->stream, s->P);
         BBPD_HIP(hipGetLastError());
     }
-    if (mixed) hipLaunchKernelGGL(k_step_end<true>, dim3(1), dim3(1024), 0, s->stream, s->P, s->S);
-    else hipLaunchKernelGGL(k_step_end<false>, dim3(1), dim3(1024), 0, s->stream, s->P, s->S);
+    const unsigned nb = (unsigned)std::max(s->S.nBodies, 1);
+    if (mixed) hipLaunchKernelGGL(k_step_end<true>, dim3(nb), dim3(1024), 0, s->stream, s->P, s->S);
+    else hipLaunchKernelGGL(k_step_end<false>, dim3(nb), dim3(1024), 0, s->stream, s->P, s->S);
     BBPD_HIP(hipGetLastError());
     return BBPD_OK;
 }
@@ -798,21 +874,38 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         }
     }
 
-    // ---- slots: Morton order of the coordinates quantised by the shortest bond
-    double xmin[3] = {1e300, 1e300, 1e300};
+    // ---- slots.  Coordinates are quantised by the shortest bond (lattice indices on a regular
+    // lattice).  Default order: tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in
+    // Morton order, lines of 64 inside a tile, so a wavefront is 64 consecutive nodes of a lattice
+    // line and the k-th neighbours of its lanes are again 64 consecutive slots (coalesced gathers).
+    // BBPD_SLOT_ORDER=morton selects a plain Morton order of the nodes instead.
+    double xmin[3] = {1e300, 1e300, 1e300}, xmax[3] = {-1e300, -1e300, -1e300};
     for (int d = 0; d < 3; d++)
-        for (int n = 0; n < nN; n++) xmin[d] = std::min(xmin[d], X[n + (size_t)nN * d]);
+        for (int n = 0; n < nN; n++) {
+            xmin[d] = std::min(xmin[d], X[n + (size_t)nN * d]);
+            xmax[d] = std::max(xmax[d], X[n + (size_t)nN * d]);
+        }
     double hq = 1e300;
     for (int k = 0; k < nB; k++) hq = std::min(hq, pr->UNDEFORMEDLENGTH[k]);
     if (!(hq > 0) || hq > 1e299) hq = 1;
+    int ax[3] = {0, 1, 2}; // ax[0] = longest axis
+    std::sort(ax, ax + 3, [&](int a, int b) { return (xmax[a] - xmin[a]) > (xmax[b] - xmin[b]); });
+    const char *ord_env = std::getenv("BBPD_SLOT_ORDER");
+    const bool morton_only = ord_env && std::string(ord_env) == "morton";
     std::vector<uint64_t> code(nN);
     for (int n = 0; n < nN; n++) {
         uint32_t q[3];
         for (int d = 0; d < 3; d++) {
-            double t = std::floor((X[n + (size_t)nN * d] - xmin[d]) / hq + 0.5);
+            double t = std::floor((X[n + (size_t)nN * ax[d]] - xmin[ax[d]]) / hq + 0.5);
             q[d] = (uint32_t)std::min(std::max(t, 0.0), 2097151.0);
         }
-        code[n] = morton3(q[0], q[1], q[2]);
+        if (morton_only) {
+            code[n] = morton3(q[0], q[1], q[2]);
+        } else {
+            uint64_t tile = morton3(q[0] >> 6, q[1] >> 1, q[2] >> 1) & ((1ull << 55) - 1);
+            uint64_t in = ((uint64_t)((q[2] & 1u) * 2u + (q[1] & 1u)) << 6) | (q[0] & 63u);
+            code[n] = (tile << 8) | in;
+        }
     }
     std::vector<int> order(nN);
     std::iota(order.begin(), order.end(), 0);
@@ -1028,6 +1121,9 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     // ---- contact bodies and observables
     StepEnd &S = s->S;
     S.nBodies = pr->nBodies;
+    // the bodies are projected concurrently (one block each), so a node may belong to one family
+    // only; the reference applies them in sequence, which is the same for disjoint families
+    std::vector<unsigned char> in_family(nN, 0);
     for (int bdy = 0; bdy < pr->nBodies; bdy++) {
         int nf = pr->nFamily[bdy];
         BBPD_REQUIRE(nf >= 0 && (nf == 0 || pr->family[bdy]), "contact family missing");
@@ -1039,6 +1135,12 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
                 bbpd::set_error("contact family holds an id outside 1..nNodes");
                 return fail(BBPD_ERR_ARG);
             }
+            if (in_family[id - 1]) {
+                bbpd::set_error("contact families overlap (node " + std::to_string(id) +
+                                "): the fused solver needs disjoint families");
+                return fail(BBPD_ERR_UNSUPPORTED);
+            }
+            in_family[id - 1] = 1;
             fs[t] = s->slot_of[id - 1];
         }
         int *fd;
@@ -1245,6 +1347,112 @@ extern "C" int bbpd_solver_get_bonds(bbpd_solver *s, double *H, double *flag, do
         if (flag) flag[k] = (w[e] & FLAG_BIT) ? 1.0 : 0.0;
         if (failout) failout[k] = (w[e] & FAILED_BIT) ? 0.0 : 1.0;
     }
+    return BBPD_OK;
+}
+
+namespace {
+
+// nodal array in reference layout (nNodes x 3, column-major) <-> three slot-ordered device arrays
+int put_nodal(bbpd_solver *s, double *const *dst, const double *src)
+{
+    const int nN = s->nN, nS = s->nSlots;
+    std::vector<double> buf((size_t)nS * 3, 0.0);
+    for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, dst[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nN; n++)
+        for (int dd = 0; dd < 3; dd++) buf[s->slot_of[n] + (size_t)nS * dd] = src[n + (size_t)nN * dd];
+    for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(dst[dd], buf.data() + (size_t)nS * dd, sizeof(double) * nS, hipMemcpyHostToDevice));
+    return BBPD_OK;
+}
+
+int get_nodal3(bbpd_solver *s, double *const *srcd, double *dst)
+{
+    const int nN = s->nN, nS = s->nSlots;
+    std::vector<double> buf((size_t)nS * 3);
+    for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, srcd[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nN; n++)
+        for (int dd = 0; dd < 3; dd++) dst[n + (size_t)nN * dd] = buf[s->slot_of[n] + (size_t)nS * dd];
+    return BBPD_OK;
+}
+
+} // namespace
+
+extern "C" int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, double *v, double *x, double *F,
+                                      double *Fp, double *H, double *flag, double *fail)
+{
+    BBPD_REQUIRE(s, "null solver");
+    BBPD_HIP(hipSetDevice(s->device));
+    BBPD_HIP(hipStreamSynchronize(s->stream));
+    DevScalars d;
+    BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
+    if (step) *step = d.step;
+    int rc = bbpd_solver_get_nodal(s, u, v, F, x);
+    if (rc) return rc;
+    if (Fp) {
+        if (s->P.Fp0) {
+            double *fp[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
+            if ((rc = get_nodal3(s, fp, Fp))) return rc;
+        } else {
+            std::memset(Fp, 0, sizeof(double) * 3 * (size_t)s->nN);
+        }
+    }
+    return bbpd_solver_get_bonds(s, H, flag, fail);
+}
+
+extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u, const double *v, const double *x,
+                                   const double *F, const double *Fp, const double *H, const double *flag,
+                                   const double *fail)
+{
+    BBPD_REQUIRE(s && step >= 0, "bad arguments");
+    BBPD_HIP(hipSetDevice(s->device));
+    BBPD_HIP(hipStreamSynchronize(s->stream));
+    const bool mixed = s->precision == BBPD_PRECISION_MIXED;
+    const int cur = (int)(step & 1);
+    double *pc[3] = {s->P.p0[cur], s->P.p1[cur], s->P.p2[cur]};
+    double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
+    double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
+    double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
+    double *fpc[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
+    int rc;
+    if (u && (rc = put_nodal(s, mixed ? pc : uc, u))) return rc;
+    if (!mixed && x && (rc = put_nodal(s, pc, x))) return rc;
+    if (v && (rc = put_nodal(s, vc, v))) return rc;
+    if (F && (rc = put_nodal(s, fc, F))) return rc;
+    if (Fp && s->P.Fp0 && (rc = put_nodal(s, fpc, Fp))) return rc;
+    const long long nE = s->nPadded;
+    if (H) {
+        for (int k = 0; k < s->nB; k++)
+            if (!(H[k] >= 0 && H[k] <= 1)) {
+                bbpd::set_error("bondSofteningFactor must lie in [0, 1]");
+                return BBPD_ERR_ARG;
+            }
+        if (mixed) {
+            std::vector<float> h(nE);
+            BBPD_HIP(hipMemcpy(h.data(), s->P.hist, 4 * nE, hipMemcpyDeviceToHost));
+            for (int k = 0; k < s->nB; k++) h[s->entry_i[k]] = h[s->entry_j[k]] = (float)H[k];
+            BBPD_HIP(hipMemcpy(s->P.hist, h.data(), 4 * nE, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> h(nE);
+            BBPD_HIP(hipMemcpy(h.data(), s->P.hist, 8 * nE, hipMemcpyDeviceToHost));
+            for (int k = 0; k < s->nB; k++) h[s->entry_i[k]] = h[s->entry_j[k]] = H[k];
+            BBPD_HIP(hipMemcpy(s->P.hist, h.data(), 8 * nE, hipMemcpyHostToDevice));
+        }
+    }
+    if (flag || fail) {
+        std::vector<uint32_t> w(nE);
+        BBPD_HIP(hipMemcpy(w.data(), s->P.nbr, sizeof(uint32_t) * nE, hipMemcpyDeviceToHost));
+        for (int k = 0; k < s->nB; k++) {
+            for (long long e : {s->entry_i[k], s->entry_j[k]}) {
+                if (flag) w[e] = (w[e] & ~FLAG_BIT) | (flag[k] == 1 ? FLAG_BIT : 0u);
+                if (fail) w[e] = (w[e] & ~FAILED_BIT) | (fail[k] == 0 ? FAILED_BIT : 0u);
+            }
+        }
+        BBPD_HIP(hipMemcpy(s->P.nbr, w.data(), sizeof(uint32_t) * nE, hipMemcpyHostToDevice));
+    }
+    DevScalars d;
+    BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
+    d.step = step;
+    BBPD_HIP(hipMemcpy(s->ds, &d, sizeof(d), hipMemcpyHostToDevice));
+    s->hist_read = step;
     return BBPD_OK;
 }
 

@@ -81,6 +81,32 @@ class Solver:
         _lib.check(self._L.bbpd_solver_get_bonds(self._h, *p), "bbpd_solver_get_bonds")
         return tuple(arrs)
 
+    def checkpoint(self) -> dict:
+        """Device state in the reference layout (see bbpd_solver_checkpoint)."""
+        nN, nB = self.case.nNodes, self.case.nBonds
+        st = {k: np.zeros((nN, 3), order="F") for k in ("u", "v", "x", "F", "Fp")}
+        st.update({k: np.zeros(nB) for k in ("H", "flag", "fail")})
+        step = C.c_int64()
+        p = [st[k].ctypes.data_as(C.c_void_p) for k in ("u", "v", "x", "F", "Fp", "H", "flag", "fail")]
+        _lib.check(self._L.bbpd_solver_checkpoint(self._h, C.byref(step), *p), "bbpd_solver_checkpoint")
+        st["step"] = step.value
+        return st
+
+    def restore(self, step: int, **arrays):
+        """Load state (keys u, v, x, F, Fp, H, flag, fail; missing keys stay unchanged)."""
+        keep = []
+
+        def ptr(k):
+            a = arrays.get(k)
+            if a is None:
+                return None
+            a = np.asfortranarray(a, dtype=np.float64) if np.ndim(a) == 2 else np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        p = [ptr(k) for k in ("u", "v", "x", "F", "Fp", "H", "flag", "fail")]
+        _lib.check(self._L.bbpd_solver_restore(self._h, int(step), *p), "bbpd_solver_restore")
+
     def history(self, cap: int = 1 << 20):
         """(reaction, deflection) of every step since the previous call."""
         r = np.zeros(cap)

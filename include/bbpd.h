@@ -219,6 +219,18 @@ int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *obs);
 int bbpd_solver_get_nodal(bbpd_solver *s, double *nodalDisplacement, double *nodalVelocity, double *nodalForce,
                           double *deformedCoordinates);
 int bbpd_solver_get_bonds(bbpd_solver *s, double *bondSofteningFactor, double *flagBondSoftening, double *fail);
+/* Checkpoint / resume of the device state (replaces the workspace save of
+ * dynamicsolverdisplacementcontrolled.m:133-141 and its reload at :46-50).  Host memory, reference
+ * layout: nodal arrays nNodes x 3 (u, v, x = deformedCoordinates, F = nodalForce,
+ * Fp = nodalForcePrevious for ADR), bond arrays nBonds in BONDLIST order (H =
+ * bondSofteningFactor, flag = flagBondSoftening, fail), `step` = steps completed
+ * (timeStepTracker - 1).  restore() takes the same arrays; NULL leaves that part unchanged.
+ * The FP32-mixed form keeps displacements, not coordinates: its x is X + u on output and ignored
+ * on input. */
+int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, double *v, double *x, double *F, double *Fp,
+                           double *H, double *flag, double *fail);
+int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u, const double *v, const double *x,
+                        const double *F, const double *Fp, const double *H, const double *flag, const double *fail);
 /* load-point reaction history: the body-0 reaction of every step since the last call */
 int bbpd_solver_history(bbpd_solver *s, double *reaction, double *deflection, int cap, int *n_out);
 /* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */

@@ -60,3 +60,13 @@ def test_struct_layouts_match_the_header(tmp_path):
         py = structs[cname]
         got = C.sizeof(py) if field == "sizeof" else getattr(py, field).offset
         assert got == int(val), (cname, field, got, val)
+
+
+def test_one_hip_runtime_per_process():
+    """libbbpd_hip binds to the HIP/HSA runtime torch already loaded (two runtimes in one process
+    leave the second without devices)."""
+    _lib.lib()
+    maps = Path("/proc/self/maps").read_text()
+    hip = set(re.findall(r"\S*libamdhip64\S*", maps))
+    hsa = set(re.findall(r"\S*libhsa-runtime64\S*", maps))
+    assert len(hip) == 1 and len(hsa) == 1, (hip, hsa)

@@ -80,15 +80,59 @@ def test_fp64_euler_cromer_bit_exact(gpu, oracle, law):
             assert obs["n_failed"] > 0
 
 
-def test_fp64_exponential_tolerance(gpu, oracle):
+def oracle_state(run):
+    return dict(u=run.u.copy(order="F"), v=run.v.copy(order="F"), x=run.x.copy(order="F"),
+                F=run.F.copy(order="F"), Fp=run.Fp.copy(order="F"), H=run.H.copy(), flag=run.flag.copy(),
+                fail=run.fail.copy())
+
+
+def test_fp64_exponential_per_step(gpu, oracle):
+    """Exponential law: from the oracle's exact state at steps 100, 200 and 299 the device step
+    gives nodal forces within 1e-12 relative (only exp() differs) and identical flags."""
     case = ec_case(_lib.LAW_EXPONENTIAL)
     print(f"case: EC exponential, {case.nBonds} bonds", flush=True)
     run = oracle.OracleRun(case)
     with Solver(case, _lib.FP64) as sol:
-        sol.step(300)
-        run.step(300)
-        compare_state(sol, run, exact=False, rtol=1e-10)
+        for k in (100, 200, 299):
+            run.step(k - run.step_count)
+            sol.restore(k, **oracle_state(run))
+            sol.step(1)
+            run.step(1)
+            compare_state(sol, run, exact=False, rtol=1e-12)
         assert sol.observables()["n_softening"] > 0
+    # free-running drift over 300 post-peak steps, printed for the record
+    run2 = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        sol.step(300)
+        run2.step(300)
+        F = sol.nodal()[2]
+        drift = np.abs(F - run2.F).max() / np.abs(run2.F).max()
+        print(f"exponential law, 300 free-running steps: max |dF|/max|F| = {drift:.2e}")
+        assert drift < 1e-6
+
+
+@pytest.mark.parametrize("integrator", [_lib.EULER_CROMER, _lib.ADR])
+def test_checkpoint_resume_bitwise(gpu, integrator):
+    """Stop at step 100, checkpoint, resume in a fresh solver: identical to an uninterrupted run."""
+    if integrator == _lib.ADR:
+        case = beam.build_beam("t", 24, 6, 10, 5e-3, law=_lib.LAW_BILINEAR, s0=1e-4, s0_perturb=0.01, sc=2e-3,
+                               integrator=_lib.ADR, iters_per_increment=50, increment=-1e-5)
+    else:
+        case = ec_case(_lib.LAW_BILINEAR)
+    with Solver(case, _lib.FP64) as a:
+        a.step(100)
+        ck = a.checkpoint()
+        a.step(100)
+        ref = a.checkpoint()
+    with Solver(case, _lib.FP64) as b:
+        step = ck.pop("step")
+        assert step == 100
+        b.restore(step, **ck)
+        b.step(100)
+        got = b.checkpoint()
+    assert got["step"] == ref["step"] == 200
+    for k in ("u", "v", "x", "F", "Fp", "H", "flag", "fail"):
+        assert same(got[k], ref[k]), k
 
 
 def test_fp64_adr_tolerance(gpu, oracle):
