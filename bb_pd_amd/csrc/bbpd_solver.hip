@@ -5,30 +5,38 @@
 //   -> nodal force (:97-101) -> time integration (:105) -> contact x3 (:108-112) -> reaction (:115).
 //
 // Device layout
-//   * Nodes live in "slots" ordered along a Morton curve of their coordinates, padded to a multiple of
-//     2048.  Nodal arrays are structure-of-arrays FP64 (one array per component).
-//   * Every bond is held from both ends in a node-centric table in SELL-64 form: the 64 slots of a
-//     slice (one wavefront) store entry k of their rows side by side, so entry loads coalesce.  A
-//     row lists its bonds in BONDLIST order, which is exactly the order the reference's serial nodal
-//     sum (calculatenodalforcesMex.c:27-44) adds them to that node: the gather is deterministic and
+//   * Nodes live in "slots": tiles of 64 x 2 x 2 lattice nodes (64 along the longest axis) in Morton
+//     order, 64-node lines inside a tile, padded to a multiple of 2048.  A wavefront is one line, and
+//     the k-th neighbours of its lanes are again 64 consecutive slots.
+//   * Node records (32 B, double-buffered): the position (deformed coordinates in the FP64 form,
+//     displacement in the mixed form) and the node's packed integer lattice coordinates.  On a
+//     lattice (X == h * k exactly, checked at set-up) the undeformed coordinates of a neighbour are
+//     rebuilt from the integers bit for bit, so a neighbour costs two 16-byte loads.  Off-lattice
+//     inputs read a second 32-byte record with X.  Other nodal arrays are structure-of-arrays FP64.
+//   * Every bond is held from both ends in a node-centric SELL-64 table: entry k of the 64 rows of a
+//     slice are 64 consecutive 16-byte records, one 16-byte load per lane.  A row lists its bonds in
+//     BONDLIST order, which is the order the reference's serial nodal sum
+//     (calculatenodalforcesMex.c:27-44) adds them to the node: the gather is deterministic and
 //     reproduces it bit for bit with no atomics.
-//   * Per entry: u32 word = neighbour slot | BONDTYPE << 28 | failed << 30 | softening flag << 31,
-//     the history bondSofteningFactor (FP64, or FP32 in the mixed form), the bond coefficient
-//     (BONDSTIFFNESS in FP64; c*V*vcf in FP32 for the mixed form) and optional per-bond s0 / sc.
-//     Both copies of a bond see the same stretch and update their history identically.
+//   * Entry record: u32 index word (neighbour slot | BONDTYPE << 28 | failed << 30 | flag << 31),
+//     u32 table indices, and the history bondSofteningFactor (FP64).  Mixed form: index word, an
+//     inline per-bond parameter (FP32), history (FP32), c*V*vcf (FP32).  FP64 BONDSTIFFNESS and
+//     per-bond s0 / sc with few distinct values live in small tables copied to LDS by each block;
+//     otherwise they are streamed from their own arrays.
 //   * UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS are recomputed from the coordinates with the
 //     reference formulas (buildhorizons.m:93-97, calculatevolumecorrectionfactors.m:16-34); set-up
 //     checks bit for bit that the given arrays equal them.
-//   * Positions are double-buffered (deformed coordinates in FP64 form, displacements in the mixed
-//     form): a step reads buffer step&1 and writes the other, so the Euler-Cromer update of a node
-//     runs in the same kernel that gathers its neighbours.
+//   * A step reads position buffer step&1 and writes the other, so the Euler-Cromer update of a node
+//     runs in the kernel that gathers its neighbours.
 #include <hip/hip_runtime.h>
 #include <algorithm>
-#include <cstdlib>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "bbpd_internal.hpp"
@@ -43,10 +51,18 @@ constexpr uint32_t TYPE_SHIFT = 28;
 constexpr uint32_t FAILED_BIT = 1u << 30;
 constexpr uint32_t FLAG_BIT = 1u << 31;
 constexpr int HIST_CAP = 1 << 20;
-constexpr int N_HIST = 4; // reaction, static reaction, deflection, CMOD
+constexpr int N_HIST = 4;      // reaction, static reaction, deflection, CMOD
+constexpr int TAB_MAX = 2048;  // entries of one LDS parameter table
+constexpr int LAT_BITS = 21;   // bits per packed lattice coordinate
+constexpr uint64_t LAT_MASK = (1ull << LAT_BITS) - 1;
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// how a per-bond quantity reaches the kernel
+enum ParamMode { PM_SCALAR = 0, PM_TABLE = 1, PM_STREAM = 2, PM_INLINE = 3 };
 
 struct DevScalars {
-    long long step; // steps completed
+    long long step;    // steps completed
     unsigned int done; // blocks of the step-end kernel finished this step
     double cn;
     double cn1, cn2;
@@ -54,19 +70,19 @@ struct DevScalars {
     double reaction_static;
 };
 
-struct Body {
-    int *fam = nullptr; // slots, in family order
-    int n = 0;
-    double cx = 0, cz = 0, R = 0;
-    int moves = 0;
+struct __align__(32) NodeRec {
+    double x, y, z; // position: deformed coordinates (FP64 form) or displacement (mixed form)
+    uint64_t q;     // packed lattice coordinates kx | ky << 21 | kz << 42
 };
 
 struct Params {
     // nodes
     int nN, nSlots;
     const double *X0, *X1, *X2;
-    double *p0[2], *p1[2], *p2[2]; // positions (x in FP64 form, u in mixed form), double-buffered
-    double *u0, *u1, *u2;          // displacement (FP64 form only)
+    NodeRec *pos[2];
+    const NodeRec *Xrec; // undeformed coordinates as records (off-lattice inputs)
+    double hx, hy, hz;   // lattice spacing per axis (lattice inputs)
+    double *u0, *u1, *u2; // displacement (FP64 form only)
     double *v0, *v1, *v2;
     double *F0, *F1, *F2;
     double *Fp0, *Fp1, *Fp2; // ADR previous force
@@ -75,10 +91,13 @@ struct Params {
     // entries
     const int *rowlen;
     const long long *slice_off;
-    uint32_t *nbr;
-    void *hist;
-    const void *coef;
-    const void *s0e, *sce;
+    uint4 *rec;
+    const double *cstr;          // FP64: BONDSTIFFNESS per entry when not tabled
+    const void *s0str, *scstr;   // per-entry s0 / sc when streamed (FP64 double, mixed float)
+    const double *tab;           // FP64 tables: c values, then parameter values
+    int ntabc, ntabp;
+    int cmode, s0mode, scmode;
+    int adr;
     // scalars
     double V, horizon, hmr, hpr, two_r; // hmr = horizon - RADIJ, hpr = horizon + RADIJ, two_r = 2*RADIJ
     double s0, s1, sc, k, alpha, denom_exp, sc_steel, ff;
@@ -147,102 +166,188 @@ __device__ inline void apply_law(T s, T s0, T sc, T s1, T kk, T alpha, T denom, 
 
 __device__ inline int remap_block(int b, int nb)
 {
-    // consecutive Morton slots stay on one XCD: hardware places block b on XCD b % 8
+    // consecutive slots stay on one XCD: hardware places block b on XCD b % 8
     int per = nb >> 3;
     return (b & 7) * per + (b >> 3);
 }
 
-// ------------------------------------------------------------------------------------------------
-// Bond kernel, plain FP64 form.  Thread = node slot.  Gathers the neighbours' deformed coordinates,
-// updates every entry's law state, sums the row-oriented bond forces in BONDLIST order, then either
-// applies the Euler-Cromer update (timeintegrationeulercromerMex.c:38-46) or, for ADR, stores F and
-// the block's Rayleigh-quotient partial sums (calculatedampingcoefficient.m:61-85).
-// ------------------------------------------------------------------------------------------------
-template <int LAW, bool S0A, bool SCA, bool ADR>
-__global__ __launch_bounds__(BLOCK) void k_bond_fp64(Params P)
+__device__ inline uint32_t *rec_word(uint4 *rec, long long e, int word)
 {
+    return reinterpret_cast<uint32_t *>(rec + e) + word;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Bond kernel.  Thread = node slot, wavefront = slice.  For each entry of its row the thread loads
+// the entry record and the neighbour's node record, updates the entry's law state, and adds the
+// row-oriented bond force; entries are summed in BONDLIST order.  After the row it either applies the
+// Euler-Cromer update (timeintegrationeulercromerMex.c:38-46) or, for ADR, stores F and the block's
+// Rayleigh-quotient partial sums (calculatedampingcoefficient.m:61-85).
+//
+// Plain FP64 form: the reference arithmetic in the reference order (bit-identical results).
+// FP32-mixed form: eta = u_m - u_n and xi = X_m - X_n are formed in FP64 and rounded, the stretch
+// uses the cancellation-free s = (2 xi.eta + eta.eta) / (|xi| (|xi + eta| + |xi|)), the law and the
+// force run in FP32 (contracted multiply-adds, hardware reciprocal and square root), the row sum and
+// all nodal state stay FP64.
+//
+// Memory: a wide load costs the address unit about as much as a narrow one, so each entry is one
+// 16-byte record and each neighbour one 32-byte record.  Two operand sets A and B alternate: while
+// entry k (A) is computed, entry k+1 (B) and the index words of k+2 and k+3 are in flight, and the
+// reverse for k+1.  The loop runs to the slice width (uniform over the wavefront); padding entries
+// are valid memory, so every load is unconditional and only the arithmetic past the row is masked.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+struct Ent {
+    uint4 r;             // entry record
+    T p0, p1;            // streamed per-bond s0, sc
+    double c;            // streamed BONDSTIFFNESS (FP64 form)
+    NodeRec nb;          // neighbour record
+    double b0, b1, b2;   // neighbour undeformed coordinates (off-lattice inputs)
+};
+
+template <bool MIXED, int LAW, bool LAT>
+__global__ __launch_bounds__(BLOCK) void k_bond(Params P)
+{
+    using T = typename std::conditional<MIXED, float, double>::type;
     __shared__ double sh[BLOCK];
+    extern __shared__ double tabs[]; // FP64 parameter tables
     const int b = remap_block(blockIdx.x, gridDim.x);
     const int n = b * BLOCK + threadIdx.x;
     const int cur = (int)(P.ds->step & 1);
-    const double *__restrict__ x0 = P.p0[cur];
-    const double *__restrict__ x1 = P.p1[cur];
-    const double *__restrict__ x2 = P.p2[cur];
-    const double *__restrict__ X0 = P.X0;
-    const double *__restrict__ X1 = P.X1;
-    const double *__restrict__ X2 = P.X2;
-    double *__restrict__ hist = (double *)P.hist;
-    const double *__restrict__ cc = (const double *)P.coef;
-    const double *__restrict__ s0e = (const double *)P.s0e;
-    const double *__restrict__ sce = (const double *)P.sce;
+    const NodeRec *__restrict__ pos = P.pos[cur];
+    uint4 *__restrict__ rec = P.rec;
+    if (!MIXED) {
+        const int nt = P.ntabc + P.ntabp;
+        for (int i = threadIdx.x; i < nt; i += BLOCK) tabs[i] = P.tab[i];
+        __syncthreads();
+    }
+    const double *tabc = tabs;
+    const double *tabp = tabs + P.ntabc;
 
     double f0 = 0, f1 = 0, f2 = 0;
     const int len = P.rowlen[n];
-    const double xn0 = x0[n], xn1 = x1[n], xn2 = x2[n];
-    const double Xn0 = X0[n], Xn1 = X1[n], Xn2 = X2[n];
-    long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
-    // software pipeline: entry kk+1's stream words and neighbour gathers are in flight while
-    // entry kk is computed (loads only move; the arithmetic and its order are unchanged)
-    uint32_t wN = 0;
-    double HN = 0, cN = 0, s0N = 0, scN = 0, xm0N = 0, xm1N = 0, xm2N = 0, Xm0N = 0, Xm1N = 0, Xm2N = 0;
-    if (len > 0) {
-        wN = P.nbr[e];
-        HN = hist[e];
-        cN = cc[e];
-        if (S0A) s0N = s0e[e];
-        if (SCA) scN = sce[e];
-        const int mN = (int)(wN & NODE_MASK);
-        xm0N = x0[mN], xm1N = x1[mN], xm2N = x2[mN];
-        Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
-    }
-    for (int kk = 0; kk < len; kk++, e += SLICE) {
-        uint32_t w = wN;
-        const uint32_t w0 = w;
-        double H = HN;
-        const double ccur = cN, s0c = s0N, scc = scN;
-        const double xm0 = xm0N, xm1 = xm1N, xm2 = xm2N, Xm0 = Xm0N, Xm1 = Xm1N, Xm2 = Xm2N;
-        if (kk + 1 < len) {
-            const long long e1 = e + SLICE;
-            wN = P.nbr[e1];
-            HN = hist[e1];
-            cN = cc[e1];
-            if (S0A) s0N = s0e[e1];
-            if (SCA) scN = sce[e1];
-            const int mN = (int)(wN & NODE_MASK);
-            xm0N = x0[mN], xm1N = x1[mN], xm2N = x2[mN];
-            Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
+    const NodeRec me = pos[n];
+    const double qn0 = me.x, qn1 = me.y, qn2 = me.z;
+    const double Xn0 = P.X0[n], Xn1 = P.X1[n], Xn2 = P.X2[n];
+    const long long sbase = P.slice_off[n >> 6];
+    const int W = (int)((P.slice_off[(n >> 6) + 1] - sbase) >> 6);
+    const long long e0 = sbase + (n & (SLICE - 1));
+    auto E = [&](int k) { return e0 + (long long)min(k, W - 1) * SLICE; };
+    // the look-ahead index word shares a cache line with its record, which is read two entries
+    // later: a normally cached load keeps that line for the record load (non-temporal here would
+    // fetch the line from memory twice)
+    auto index = [&](int k) { return *rec_word(rec, E(k), 0); };
+    auto load = [&](Ent<T> &d, int k, uint32_t w) {
+        const long long e = E(k);
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(rec + e));
+        d.r = make_uint4(v.x, v.y, v.z, v.w);
+        if (P.s0mode == PM_STREAM) d.p0 = __builtin_nontemporal_load((const T *)P.s0str + e);
+        if (P.scmode == PM_STREAM) d.p1 = __builtin_nontemporal_load((const T *)P.scstr + e);
+        if (!MIXED && P.cmode == PM_STREAM) d.c = __builtin_nontemporal_load(P.cstr + e);
+        const int m = (int)(w & NODE_MASK);
+        d.nb = pos[m];
+        if (!LAT) {
+            const NodeRec xr = P.Xrec[m];
+            d.b0 = xr.x, d.b1 = xr.y, d.b2 = xr.z;
         }
-        // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n): exact negation of the
-        // reference's x_j - x_i at the j end, so |.|, s and the force magnitude are identical
-        const double dx = xm0 - xn0, dy = xm1 - xn1, dz = xm2 - xn2;
-        const double l = sqrt(dx * dx + dy * dy + dz * dz);
-        const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
-        const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
-        const double s = (l - L0) / L0;
-        const double H0 = H;
-        const double s0 = S0A ? s0c : P.s0;
-        const double sc = SCA ? scc : P.sc;
-        apply_law<LAW, double>(s, s0, sc, P.s1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
-        if (w != w0) P.nbr[e] = w;
-        if (__double_as_longlong(H) != __double_as_longlong(H0)) hist[e] = H;
-        // calculatevolumecorrectionfactors.m:20-32
-        double vcf;
-        if (L0 <= P.hmr) vcf = 1;
-        else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
-        else vcf = 0;
-        // calculatebondforcesMex.c:34-40 with stretchPlastic = 0 (s - 0 == s exactly)
-        const double fail = (w & FAILED_BIT) ? 0.0 : 1.0;
-        const double p = fail * ccur * (1 - H) * s * P.V * vcf;
-        double gx = p * (dx / l), gy = p * (dy / l), gz = p * (dz / l);
-        if (gx != gx) gx = 0.0;
-        if (gy != gy) gy = 0.0;
-        if (gz != gz) gz = 0.0;
-        f0 += gx;
-        f1 += gy;
-        f2 += gz;
+    };
+    auto compute = [&](const Ent<T> &d, long long e) {
+        uint32_t w = d.r.x;
+        double Xm0, Xm1, Xm2;
+        if (LAT) {
+            const uint64_t q = d.nb.q;
+            Xm0 = P.hx * (double)(int)(q & LAT_MASK);
+            Xm1 = P.hy * (double)(int)((q >> LAT_BITS) & LAT_MASK);
+            Xm2 = P.hz * (double)(int)(q >> (2 * LAT_BITS));
+        } else {
+            Xm0 = d.b0, Xm1 = d.b1, Xm2 = d.b2;
+        }
+        if constexpr (!MIXED) {
+            double H = __hiloint2double((int)d.r.w, (int)d.r.z);
+            const double H0 = H;
+            const double c = P.cmode == PM_TABLE ? tabc[d.r.y & 0xFFFFu] : d.c;
+            const double s0 = P.s0mode == PM_TABLE ? tabp[d.r.y >> 16] : (P.s0mode == PM_STREAM ? d.p0 : P.s0);
+            const double sc = P.scmode == PM_TABLE ? tabp[d.r.y >> 16] : (P.scmode == PM_STREAM ? d.p1 : P.sc);
+            // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n): exact negation of the
+            // reference's x_j - x_i at the j end, so |.|, s and the force magnitude are identical
+            const double dx = d.nb.x - qn0, dy = d.nb.y - qn1, dz = d.nb.z - qn2;
+            const double l = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
+            const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+            const double s = (l - L0) / L0;
+            apply_law<LAW, double>(s, s0, sc, P.s1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
+            // calculatevolumecorrectionfactors.m:20-32
+            double vcf;
+            if (L0 <= P.hmr) vcf = 1;
+            else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
+            else vcf = 0;
+            // calculatebondforcesMex.c:34-40 with stretchPlastic = 0 (s - 0 == s exactly)
+            const double fail = (w & FAILED_BIT) ? 0.0 : 1.0;
+            const double p = fail * c * (1 - H) * s * P.V * vcf;
+            // dx/l, dy/l, dz/l: with rl = 1/l correctly rounded, q = dx*rl, r = dx - l*q (exact by
+            // FMA) and q' = q + r*rl rounded once is the correctly rounded quotient (Markstein's
+            // theorem; no quotient here is subnormal), so this equals the IEEE division bit for bit
+            const double rl = 1.0 / l;
+            double qx = dx * rl, qy = dy * rl, qz = dz * rl;
+            qx = fma(fma(-l, qx, dx), rl, qx);
+            qy = fma(fma(-l, qy, dy), rl, qy);
+            qz = fma(fma(-l, qz, dz), rl, qz);
+            double gx = p * qx, gy = p * qy, gz = p * qz;
+            if (gx != gx) gx = 0.0;
+            if (gy != gy) gy = 0.0;
+            if (gz != gz) gz = 0.0;
+            f0 += gx;
+            f1 += gy;
+            f2 += gz;
+            if (__double_as_longlong(H) != __double_as_longlong(H0))
+                *reinterpret_cast<double *>(rec_word(rec, e, 2)) = H;
+        } else {
+            float H = __uint_as_float(d.r.z);
+            const float H0 = H;
+            const float kap = __uint_as_float(d.r.w);
+            const float s0 = P.s0mode == PM_INLINE ? __uint_as_float(d.r.y)
+                                                   : (P.s0mode == PM_STREAM ? d.p0 : (float)P.s0);
+            const float sc = P.scmode == PM_INLINE ? __uint_as_float(d.r.y)
+                                                   : (P.scmode == PM_STREAM ? d.p1 : (float)P.sc);
+            const float xi0 = (float)(Xm0 - Xn0), xi1 = (float)(Xm1 - Xn1), xi2 = (float)(Xm2 - Xn2);
+            const float et0 = (float)(d.nb.x - qn0), et1 = (float)(d.nb.y - qn1), et2 = (float)(d.nb.z - qn2);
+            const float y0 = xi0 + et0, y1 = xi1 + et1, y2 = xi2 + et2;
+            const float L0 = __builtin_amdgcn_sqrtf(fmaf(xi0, xi0, fmaf(xi1, xi1, xi2 * xi2)));
+            const float l = __builtin_amdgcn_sqrtf(fmaf(y0, y0, fmaf(y1, y1, y2 * y2)));
+            const float num =
+                fmaf(2.0f, fmaf(xi0, et0, fmaf(xi1, et1, xi2 * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
+            const float rl = __builtin_amdgcn_rcpf(l);
+            const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
+            apply_law<LAW, float>(s, s0, sc, (float)P.s1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
+                                  (float)P.sc_steel, (float)P.ff, H, w);
+            const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
+            const float p = fail * kap * (1.0f - H) * s * rl;
+            float gx = p * y0, gy = p * y1, gz = p * y2;
+            if (gx != gx) gx = 0.0f;
+            if (gy != gy) gy = 0.0f;
+            if (gz != gz) gz = 0.0f;
+            f0 += (double)gx;
+            f1 += (double)gy;
+            f2 += (double)gz;
+            if (__float_as_uint(H) != __float_as_uint(H0)) *rec_word(rec, e, 2) = __float_as_uint(H);
+        }
+        if (w != d.r.x) *rec_word(rec, e, 0) = w;
+    };
+    if (W > 0) {
+        Ent<T> A, B;
+        uint32_t iB = index(1);
+        load(A, 0, index(0));
+        for (int k = 0; k < W; k += 2) {
+            const uint32_t iA2 = index(k + 2);
+            const uint32_t iB2 = index(k + 3);
+            load(B, k + 1, iB);
+            if (k < len) compute(A, E(k));
+            load(A, k + 2, iA2);
+            if (k + 1 < len) compute(B, E(k + 1));
+            iB = iB2;
+        }
     }
     const bool real = n < P.nN;
-    if (!ADR) {
+    if (!P.adr) {
         if (!real) return;
         const int nxt = cur ^ 1;
         P.F0[n] = f0;
@@ -252,151 +357,28 @@ __global__ __launch_bounds__(BLOCK) void k_bond_fp64(Params P)
         const double r = P.rho[n];
         double *uu[3] = {P.u0, P.u1, P.u2};
         double *vv[3] = {P.v0, P.v1, P.v2};
-        double *xo[3] = {P.p0[nxt], P.p1[nxt], P.p2[nxt]};
         const double Xn[3] = {Xn0, Xn1, Xn2};
+        const double qn[3] = {qn0, qn1, qn2};
         const double ff[3] = {f0, f1, f2};
+        double out[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             double a = (ff[d] - P.D * vv[d][n]) / r;
             if (cf & (1u << d)) a = 0;
             const double nv = vv[d][n] + (a * P.DT);
-            const double nu = uu[d][n] + (nv * P.DT);
             vv[d][n] = nv;
-            uu[d][n] = nu;
-            xo[d][n] = Xn[d] + nu;
-        }
-    } else {
-        double c1 = 0, c2 = 0;
-        if (real) {
-            P.F0[n] = f0;
-            P.F1[n] = f1;
-            P.F2[n] = f2;
-            const double mi = P.mass[n];
-            const double ff[3] = {f0, f1, f2};
-            const double fp[3] = {P.Fp0[n], P.Fp1[n], P.Fp2[n]};
-            const double vp[3] = {P.v0[n], P.v1[n], P.v2[n]};
-            const double uq[3] = {P.u0[n], P.u1[n], P.u2[n]};
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                if (vp[d] != 0) c1 = c1 - (uq[d] * uq[d]) * ((ff[d] / mi - fp[d] / mi) / (P.DT * vp[d]));
-                c2 = c2 + uq[d] * uq[d];
+            if constexpr (MIXED) {
+                out[d] = qn[d] + (nv * P.DT);
+            } else {
+                const double nu = uu[d][n] + (nv * P.DT);
+                uu[d][n] = nu;
+                out[d] = Xn[d] + nu;
             }
         }
-        const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
-        const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
-        if (threadIdx.x == 0) {
-            P.partials[2 * b] = s1;
-            P.partials[2 * b + 1] = s2;
-        }
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// Bond kernel, FP32-mixed form.  Relative displacement eta = u_m - u_n and bond vector
-// xi = X_m - X_n are formed in FP64 and rounded; the stretch uses the cancellation-free form
-//   s = (2 xi.eta + eta.eta) / (|xi| (|xi + eta| + |xi|)),
-// the law and the force run in FP32, the row sum and all nodal state stay FP64.
-// ------------------------------------------------------------------------------------------------
-template <int LAW, bool S0A, bool SCA, bool ADR>
-__global__ __launch_bounds__(BLOCK) void k_bond_mixed(Params P)
-{
-    __shared__ double sh[BLOCK];
-    const int b = remap_block(blockIdx.x, gridDim.x);
-    const int n = b * BLOCK + threadIdx.x;
-    const int cur = (int)(P.ds->step & 1);
-    const double *__restrict__ u0 = P.p0[cur];
-    const double *__restrict__ u1 = P.p1[cur];
-    const double *__restrict__ u2 = P.p2[cur];
-    const double *__restrict__ X0 = P.X0;
-    const double *__restrict__ X1 = P.X1;
-    const double *__restrict__ X2 = P.X2;
-    float *__restrict__ hist = (float *)P.hist;
-    const float *__restrict__ kap = (const float *)P.coef;
-    const float *__restrict__ s0e = (const float *)P.s0e;
-    const float *__restrict__ sce = (const float *)P.sce;
-    const float s0c = (float)P.s0, scc = (float)P.sc, s1c = (float)P.s1, kc = (float)P.k, ac = (float)P.alpha;
-    const float dc = (float)P.denom_exp, sstc = (float)P.sc_steel, ffc = (float)P.ff;
-
-    double f0 = 0, f1 = 0, f2 = 0;
-    const int len = P.rowlen[n];
-    const double un0 = u0[n], un1 = u1[n], un2 = u2[n];
-    const double Xn0 = X0[n], Xn1 = X1[n], Xn2 = X2[n];
-    long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
-    // software pipeline as in the FP64 kernel: entry kk+1 is loaded while entry kk is computed
-    uint32_t wN = 0;
-    float HN = 0.f, kN = 0.f, s0N = 0.f, scN = 0.f;
-    double um0N = 0, um1N = 0, um2N = 0, Xm0N = 0, Xm1N = 0, Xm2N = 0;
-    if (len > 0) {
-        wN = P.nbr[e];
-        HN = hist[e];
-        kN = kap[e];
-        if (S0A) s0N = s0e[e];
-        if (SCA) scN = sce[e];
-        const int mN = (int)(wN & NODE_MASK);
-        um0N = u0[mN], um1N = u1[mN], um2N = u2[mN];
-        Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
-    }
-    for (int kk = 0; kk < len; kk++, e += SLICE) {
-        uint32_t w = wN;
-        const uint32_t w0 = w;
-        float H = HN;
-        const float kcur = kN, s0cur = s0N, sccur = scN;
-        const double um0 = um0N, um1 = um1N, um2 = um2N, Xm0 = Xm0N, Xm1 = Xm1N, Xm2 = Xm2N;
-        if (kk + 1 < len) {
-            const long long e1 = e + SLICE;
-            wN = P.nbr[e1];
-            HN = hist[e1];
-            kN = kap[e1];
-            if (S0A) s0N = s0e[e1];
-            if (SCA) scN = sce[e1];
-            const int mN = (int)(wN & NODE_MASK);
-            um0N = u0[mN], um1N = u1[mN], um2N = u2[mN];
-            Xm0N = X0[mN], Xm1N = X1[mN], Xm2N = X2[mN];
-        }
-        const float xi0 = (float)(Xm0 - Xn0), xi1 = (float)(Xm1 - Xn1), xi2 = (float)(Xm2 - Xn2);
-        const float et0 = (float)(um0 - un0), et1 = (float)(um1 - un1), et2 = (float)(um2 - un2);
-        const float y0 = xi0 + et0, y1 = xi1 + et1, y2 = xi2 + et2;
-        const float L0 = sqrtf(xi0 * xi0 + xi1 * xi1 + xi2 * xi2);
-        const float l = sqrtf(y0 * y0 + y1 * y1 + y2 * y2);
-        const float num = 2.0f * (xi0 * et0 + xi1 * et1 + xi2 * et2) + (et0 * et0 + et1 * et1 + et2 * et2);
-        const float s = num / (L0 * (l + L0));
-        const float H0 = H;
-        const float s0 = S0A ? s0cur : s0c;
-        const float sc = SCA ? sccur : scc;
-        apply_law<LAW, float>(s, s0, sc, s1c, kc, ac, dc, sstc, ffc, H, w);
-        if (w != w0) P.nbr[e] = w;
-        if (__float_as_uint(H) != __float_as_uint(H0)) hist[e] = H;
-        const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
-        const float p = fail * kcur * (1.0f - H) * s / l;
-        float gx = p * y0, gy = p * y1, gz = p * y2;
-        if (gx != gx) gx = 0.0f;
-        if (gy != gy) gy = 0.0f;
-        if (gz != gz) gz = 0.0f;
-        f0 += (double)gx;
-        f1 += (double)gy;
-        f2 += (double)gz;
-    }
-    const bool real = n < P.nN;
-    if (!ADR) {
-        if (!real) return;
-        const int nxt = cur ^ 1;
-        P.F0[n] = f0;
-        P.F1[n] = f1;
-        P.F2[n] = f2;
-        const unsigned char cf = P.cons[n];
-        const double r = P.rho[n];
-        double *vv[3] = {P.v0, P.v1, P.v2};
-        double *uo[3] = {P.p0[nxt], P.p1[nxt], P.p2[nxt]};
-        const double un[3] = {un0, un1, un2};
-        const double ff[3] = {f0, f1, f2};
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double a = (ff[d] - P.D * vv[d][n]) / r;
-            if (cf & (1u << d)) a = 0;
-            const double nv = vv[d][n] + (a * P.DT);
-            vv[d][n] = nv;
-            uo[d][n] = un[d] + (nv * P.DT);
-        }
+        NodeRec *po = P.pos[nxt] + n;
+        po->x = out[0];
+        po->y = out[1];
+        po->z = out[2];
     } else {
         double c1 = 0, c2 = 0;
         if (real) {
@@ -407,7 +389,12 @@ __global__ __launch_bounds__(BLOCK) void k_bond_mixed(Params P)
             const double ff[3] = {f0, f1, f2};
             const double fp[3] = {P.Fp0[n], P.Fp1[n], P.Fp2[n]};
             const double vp[3] = {P.v0[n], P.v1[n], P.v2[n]};
-            const double uq[3] = {un0, un1, un2};
+            double uq[3];
+            if constexpr (MIXED) {
+                uq[0] = qn0, uq[1] = qn1, uq[2] = qn2;
+            } else {
+                uq[0] = P.u0[n], uq[1] = P.u1[n], uq[2] = P.u2[n];
+            }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 if (vp[d] != 0) c1 = c1 - (uq[d] * uq[d]) * ((ff[d] / mi - fp[d] / mi) / (P.DT * vp[d]));
@@ -461,15 +448,16 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
     double *FP[3] = {P.Fp0, P.Fp1, P.Fp2};
     const double *XX[3] = {P.X0, P.X1, P.X2};
     double *uu[3] = {P.u0, P.u1, P.u2};
-    double *pc[3] = {P.p0[cur], P.p1[cur], P.p2[cur]};
-    double *po[3] = {P.p0[nxt], P.p1[nxt], P.p2[nxt]};
+    const NodeRec pc = P.pos[cur][n];
+    const double pcv[3] = {pc.x, pc.y, pc.z};
+    double out[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double f = FF[d][n];
         double nv;
         if (it == 1) nv = 0.5 * DT * minv * f;
         else nv = (((2 - cn * DT) * vv[d][n]) + (2 * DT * minv * f)) / (2 + cn * DT);
-        const double uold = MIXED ? pc[d][n] : uu[d][n];
+        const double uold = MIXED ? pcv[d] : uu[d][n];
         double nu = uold + (DT * nv);
         if (cf & (1u << d)) {
             nu = 0;
@@ -477,20 +465,22 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
         }
         vv[d][n] = nv;
         if (MIXED) {
-            po[d][n] = nu;
+            out[d] = nu;
         } else {
             uu[d][n] = nu;
-            po[d][n] = XX[d][n] + nu;
+            out[d] = XX[d][n] + nu;
         }
         FP[d][n] = f;
     }
+    NodeRec *po = P.pos[nxt] + n;
+    po->x = out[0];
+    po->y = out[1];
+    po->z = out[2];
 }
 
 // ------------------------------------------------------------------------------------------------
-// Step end: contact with every rigid body in order (calculatecontactforce.m:62-121), the load
-// schedule (smoothstepdata.m:40-42 or a staircase of increments), the observables of the step and
-// the step counter.  One block; family members are projected in parallel and each body's reaction
-// is summed in family order by one thread.
+// Step end: contact with every rigid body (calculatecontactforce.m:62-121), the load schedule
+// (smoothstepdata.m:40-42 or a staircase of increments), the step's observables and the step counter.
 // ------------------------------------------------------------------------------------------------
 struct StepEnd {
     int nBodies;
@@ -525,7 +515,7 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
     __shared__ int last_block;
     const int nxt = (int)((P.ds->step + 1) & 1);
     const long long it = P.ds->step + 1;
-    double *pos[3] = {P.p0[nxt], P.p1[nxt], P.p2[nxt]};
+    NodeRec *pos = P.pos[nxt];
     double *uu[3] = {P.u0, P.u1, P.u2};
     double *vv[3] = {P.v0, P.v1, P.v2};
     const double *XX[3] = {P.X0, P.X1, P.X2};
@@ -542,9 +532,11 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
             double d_z = 0;
             if (t < nf) {
                 const int i = S.fam[bdy][t];
+                NodeRec pr = pos[i];
+                double pv[3] = {pr.x, pr.y, pr.z};
                 double xc[3];
 #pragma unroll
-                for (int d = 0; d < 3; d++) xc[d] = MIXED ? XX[d][i] + pos[d][i] : pos[d][i];
+                for (int d = 0; d < 3; d++) xc[d] = MIXED ? XX[d][i] + pv[d] : pv[d];
                 const double ddx = xc[0] - cx, ddz = xc[2] - czm;
                 const double dist = sqrt((ddx * ddx) + (ddz * ddz));
                 if (dist < R) {
@@ -553,19 +545,22 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
                     xc[2] = czm + (ddz / dist) * R;
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
-                        const double uo = MIXED ? pos[d][i] : uu[d][i];
+                        const double uo = MIXED ? pv[d] : uu[d][i];
                         const double vo = vv[d][i];
                         const double nu = xc[d] - XX[d][i];
                         const double nv = (nu - uo) / P.DT;
                         if (MIXED) {
-                            pos[d][i] = nu;
+                            pv[d] = nu;
                         } else {
                             uu[d][i] = nu;
-                            pos[d][i] = xc[d];
+                            pv[d] = xc[d];
                         }
                         vv[d][i] = nv;
                         if (d == 2) d_z = -1 * P.rho[i] * (nv - vo) / P.DT * P.V;
                     }
+                    pos[i].x = pv[0];
+                    pos[i].y = pv[1];
+                    pos[i].z = pv[2];
                 }
             }
             // compact the members in contact, in family order, then sum them in that order
@@ -602,10 +597,16 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
     if (last_block && threadIdx.x == 0) {
         __threadfence();
         volatile DevScalars *vds = P.ds;
-        const volatile double *uz = MIXED ? pos[2] : uu[2];
-        const volatile double *ux = MIXED ? pos[0] : uu[0];
-        const double defl = S.ref_slot >= 0 ? uz[S.ref_slot] : 0.0;
-        const double cmod = (S.cmod1 >= 0 && S.cmod2 >= 0) ? ux[S.cmod2] - ux[S.cmod1] : 0.0;
+        auto comp = [&](int slot, int d) -> double {
+            if (MIXED) {
+                const volatile double *pp = reinterpret_cast<const volatile double *>(pos + slot);
+                return pp[d];
+            }
+            const volatile double *pu = uu[d];
+            return pu[slot];
+        };
+        const double defl = S.ref_slot >= 0 ? comp(S.ref_slot, 2) : 0.0;
+        const double cmod = (S.cmod1 >= 0 && S.cmod2 >= 0) ? comp(S.cmod2, 0) - comp(S.cmod1, 0) : 0.0;
         double *h = S.hist + (size_t)(it - 1) % HIST_CAP * N_HIST;
         h[0] = S.nBodies > 0 ? vds->reaction[0] : 0.0;
         h[1] = vds->reaction_static;
@@ -626,8 +627,9 @@ __global__ __launch_bounds__(BLOCK) void k_count(Params P, unsigned long long *o
         const int len = P.rowlen[n];
         long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
         for (int kk = 0; kk < len; kk++, e += SLICE) {
-            uint32_t w = P.nbr[e];
-            bool one = MIXED ? (((const float *)P.hist)[e] == 1.0f) : (((const double *)P.hist)[e] == 1.0);
+            const uint4 r = P.rec[e];
+            const uint32_t w = r.x;
+            const bool one = MIXED ? (__uint_as_float(r.z) == 1.0f) : (__hiloint2double((int)r.w, (int)r.z) == 1.0);
             cs += (w & FLAG_BIT) ? 1 : 0;
             cf += (w & FAILED_BIT) ? 1 : 0;
             cb += ((w & FAILED_BIT) || one) ? 1 : 0;
@@ -671,20 +673,37 @@ int dalloc(T **p, size_t n, std::vector<void *> &owned)
     return BBPD_OK;
 }
 
+double bits_to_double(uint32_t lo, uint32_t hi)
+{
+    uint64_t v = ((uint64_t)hi << 32) | lo;
+    double d;
+    std::memcpy(&d, &v, 8);
+    return d;
+}
+
+void double_to_bits(double d, uint32_t &lo, uint32_t &hi)
+{
+    uint64_t v;
+    std::memcpy(&v, &d, 8);
+    lo = (uint32_t)v;
+    hi = (uint32_t)(v >> 32);
+}
+
 } // namespace
 
 struct bbpd_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     int precision = 0, law = 0, integrator = 0;
-    bool s0arr = false, scarr = false;
+    bool lattice = false;
     int nN = 0, nB = 0, nSlots = 0, nSlices = 0;
     long long nEntries = 0, nPadded = 0;
-    std::vector<int> slot_of;               // original node (0-based) -> slot
+    std::vector<int> slot_of;                // original node (0-based) -> slot
     std::vector<long long> entry_i, entry_j; // bond -> entry index at its two ends
     std::vector<void *> owned;
     Params P{};
     StepEnd S{};
+    size_t lds_bytes = 0;
     DevScalars *ds = nullptr;
     unsigned long long *counts = nullptr;
     double *hist = nullptr;
@@ -705,40 +724,27 @@ namespace {
 
 using KernelFn = void (*)(Params);
 
-template <bool MIXED, int LAW, bool S0A, bool SCA, bool ADR>
-KernelFn pick4()
-{
-    if (MIXED) return k_bond_mixed<LAW, S0A, SCA, ADR>;
-    return k_bond_fp64<LAW, S0A, SCA, ADR>;
-}
-
 template <bool MIXED, int LAW>
-KernelFn pick3(bool s0a, bool sca, bool adr)
+KernelFn pick_lat(bool lat)
 {
-    if (s0a) {
-        if (sca) return adr ? pick4<MIXED, LAW, true, true, true>() : pick4<MIXED, LAW, true, true, false>();
-        return adr ? pick4<MIXED, LAW, true, false, true>() : pick4<MIXED, LAW, true, false, false>();
-    }
-    if (sca) return adr ? pick4<MIXED, LAW, false, true, true>() : pick4<MIXED, LAW, false, true, false>();
-    return adr ? pick4<MIXED, LAW, false, false, true>() : pick4<MIXED, LAW, false, false, false>();
+    return lat ? k_bond<MIXED, LAW, true> : k_bond<MIXED, LAW, false>;
 }
 
 template <bool MIXED>
-KernelFn pick2(int law, bool s0a, bool sca, bool adr)
+KernelFn pick_law(int law, bool lat)
 {
     switch (law) {
-    case BBPD_LAW_LINEAR: return pick3<MIXED, BBPD_LAW_LINEAR>(s0a, sca, adr);
-    case BBPD_LAW_BILINEAR: return pick3<MIXED, BBPD_LAW_BILINEAR>(s0a, sca, adr);
-    case BBPD_LAW_TRILINEAR: return pick3<MIXED, BBPD_LAW_TRILINEAR>(s0a, sca, adr);
-    default: return pick3<MIXED, BBPD_LAW_EXPONENTIAL>(s0a, sca, adr);
+    case BBPD_LAW_LINEAR: return pick_lat<MIXED, BBPD_LAW_LINEAR>(lat);
+    case BBPD_LAW_BILINEAR: return pick_lat<MIXED, BBPD_LAW_BILINEAR>(lat);
+    case BBPD_LAW_TRILINEAR: return pick_lat<MIXED, BBPD_LAW_TRILINEAR>(lat);
+    default: return pick_lat<MIXED, BBPD_LAW_EXPONENTIAL>(lat);
     }
 }
 
 KernelFn pick_bond_kernel(const bbpd_solver *s)
 {
-    bool adr = s->integrator == BBPD_INTEGRATOR_ADR;
-    if (s->precision == BBPD_PRECISION_MIXED) return pick2<true>(s->law, s->s0arr, s->scarr, adr);
-    return pick2<false>(s->law, s->s0arr, s->scarr, adr);
+    if (s->precision == BBPD_PRECISION_MIXED) return pick_law<true>(s->law, s->lattice);
+    return pick_law<false>(s->law, s->lattice);
 }
 
 int launch_step(bbpd_solver *s, bool allow_timing)
@@ -760,7 +766,7 @@ int launch_step(bbpd_solver *s, bool allow_timing)
         }
         BBPD_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     }
-    hipLaunchKernelGGL(kb, dim3(grid), dim3(BLOCK), 0, s->stream, s->P);
+    hipLaunchKernelGGL(kb, dim3(grid), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
     BBPD_HIP(hipGetLastError());
     if (timed) {
         BBPD_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
@@ -788,6 +794,26 @@ bool bits_equal(double a, double b)
     std::memcpy(&x, &a, 8);
     std::memcpy(&y, &b, 8);
     return x == y;
+}
+
+// distinct values of a per-bond array, as a table and per-bond indices; false if more than cap
+bool make_table(const double *a, int n, int cap, std::vector<double> &tab, std::vector<uint32_t> &idx)
+{
+    std::unordered_map<uint64_t, uint32_t> m;
+    tab.clear();
+    idx.assign(n, 0);
+    for (int k = 0; k < n; k++) {
+        uint64_t key;
+        std::memcpy(&key, &a[k], 8);
+        auto it = m.find(key);
+        if (it == m.end()) {
+            if ((int)tab.size() >= cap) return false;
+            it = m.emplace(key, (uint32_t)tab.size()).first;
+            tab.push_back(a[k]);
+        }
+        idx[k] = it->second;
+    }
+    return true;
 }
 
 } // namespace
@@ -848,7 +874,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         }
         bi[k] = i - 1;
         bj[k] = j - 1;
-        double ex = X[i - 1] - X[j - 1], ey = X[i - 1 + nN] - X[j - 1 + nN], ez = X[i - 1 + 2 * (size_t)nN] - X[j - 1 + 2 * (size_t)nN];
+        double ex = X[i - 1] - X[j - 1], ey = X[i - 1 + nN] - X[j - 1 + nN],
+               ez = X[i - 1 + 2 * (size_t)nN] - X[j - 1 + 2 * (size_t)nN];
         double L0 = std::sqrt((ex * ex + ey * ey) + ez * ez);
         if (!bits_equal(L0, pr->UNDEFORMEDLENGTH[k])) {
             bbpd::set_error("UNDEFORMEDLENGTH(" + std::to_string(k + 1) +
@@ -874,11 +901,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         }
     }
 
-    // ---- slots.  Coordinates are quantised by the shortest bond (lattice indices on a regular
-    // lattice).  Default order: tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in
-    // Morton order, lines of 64 inside a tile, so a wavefront is 64 consecutive nodes of a lattice
-    // line and the k-th neighbours of its lanes are again 64 consecutive slots (coalesced gathers).
-    // BBPD_SLOT_ORDER=morton selects a plain Morton order of the nodes instead.
+    // ---- lattice detection: X == h * k exactly with small non-negative integers k, per axis
     double xmin[3] = {1e300, 1e300, 1e300}, xmax[3] = {-1e300, -1e300, -1e300};
     for (int d = 0; d < 3; d++)
         for (int n = 0; n < nN; n++) {
@@ -888,10 +911,26 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     double hq = 1e300;
     for (int k = 0; k < nB; k++) hq = std::min(hq, pr->UNDEFORMEDLENGTH[k]);
     if (!(hq > 0) || hq > 1e299) hq = 1;
+    double hlat[3] = {0, 0, 0};
+    bool lattice = true;
+    for (int d = 0; d < 3 && lattice; d++) {
+        // the smallest coordinate is h * k0 with k0 = round(xmin / shortest bond)
+        double k0 = std::floor(xmin[d] / hq + 0.5);
+        double h = (k0 >= 1) ? xmin[d] / k0 : hq;
+        hlat[d] = h;
+        for (int n = 0; n < nN && lattice; n++) {
+            double xv = X[n + (size_t)nN * d];
+            double kk = std::floor(xv / h + 0.5);
+            if (!(kk >= 0 && kk < (double)(1 << LAT_BITS)) || !bits_equal(h * (double)(int)kk, xv)) lattice = false;
+        }
+    }
+    s->lattice = lattice;
+
+    // ---- slots.  Tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in Morton order,
+    // lines of 64 inside a tile, so a wavefront is 64 consecutive nodes of a lattice line and the
+    // k-th neighbours of its lanes are again 64 consecutive slots.
     int ax[3] = {0, 1, 2}; // ax[0] = longest axis
     std::sort(ax, ax + 3, [&](int a, int b) { return (xmax[a] - xmin[a]) > (xmax[b] - xmin[b]); });
-    const char *ord_env = std::getenv("BBPD_SLOT_ORDER");
-    const bool morton_only = ord_env && std::string(ord_env) == "morton";
     std::vector<uint64_t> code(nN);
     for (int n = 0; n < nN; n++) {
         uint32_t q[3];
@@ -899,13 +938,9 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
             double t = std::floor((X[n + (size_t)nN * ax[d]] - xmin[ax[d]]) / hq + 0.5);
             q[d] = (uint32_t)std::min(std::max(t, 0.0), 2097151.0);
         }
-        if (morton_only) {
-            code[n] = morton3(q[0], q[1], q[2]);
-        } else {
-            uint64_t tile = morton3(q[0] >> 6, q[1] >> 1, q[2] >> 1) & ((1ull << 55) - 1);
-            uint64_t in = ((uint64_t)((q[2] & 1u) * 2u + (q[1] & 1u)) << 6) | (q[0] & 63u);
-            code[n] = (tile << 8) | in;
-        }
+        uint64_t tile = morton3(q[0] >> 6, q[1] >> 1, q[2] >> 1) & ((1ull << 55) - 1);
+        uint64_t in = ((uint64_t)((q[2] & 1u) * 2u + (q[1] & 1u)) << 6) | (q[0] & 63u);
+        code[n] = (tile << 8) | in;
     }
     std::vector<int> order(nN);
     std::iota(order.begin(), order.end(), 0);
@@ -935,57 +970,89 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     s->entry_i.assign(nB, 0);
     s->entry_j.assign(nB, 0);
     auto entry_at = [&](int slot, int kk) { return slice_off[slot / SLICE] + (long long)kk * SLICE + (slot % SLICE); };
-    std::vector<uint32_t> h_nbr(nE, 0);
+
+    // ---- per-bond quantities: tables (FP64 form), inline (mixed form) or streams
+    int cmode = PM_STREAM, s0mode = pr->s0 ? PM_STREAM : PM_SCALAR, scmode = pr->sc ? PM_STREAM : PM_SCALAR;
+    std::vector<double> tabc, tabp;
+    std::vector<uint32_t> cidx, pidx;
+    if (!mixed) {
+        if (make_table(pr->BONDSTIFFNESS, nB, TAB_MAX, tabc, cidx)) cmode = PM_TABLE;
+        // one per-bond parameter may use the second table
+        if (cmode == PM_TABLE && (pr->s0 != nullptr) != (pr->sc != nullptr)) {
+            const double *pa = pr->s0 ? pr->s0 : pr->sc;
+            if (make_table(pa, nB, TAB_MAX, tabp, pidx)) {
+                if (pr->s0) s0mode = PM_TABLE;
+                else scmode = PM_TABLE;
+            }
+        }
+        if (cmode != PM_TABLE) tabc.clear();
+        if (s0mode != PM_TABLE && scmode != PM_TABLE) tabp.clear();
+    } else {
+        // the first per-bond parameter travels inside the record; a second one is streamed
+        if (pr->s0) s0mode = PM_INLINE;
+        else if (pr->sc) scmode = PM_INLINE;
+    }
+    std::vector<uint4> h_rec(nE);
+    std::memset(h_rec.data(), 0, sizeof(uint4) * nE);
+    std::vector<double> h_c64, h_s064, h_sc64;
+    std::vector<float> h_s032, h_sc032;
+    if (!mixed && cmode == PM_STREAM) h_c64.assign(nE, 0.0);
+    if (s0mode == PM_STREAM) {
+        if (mixed) h_s032.assign(nE, 0.f);
+        else h_s064.assign(nE, 0.0);
+    }
+    if (scmode == PM_STREAM) {
+        if (mixed) h_sc032.assign(nE, 0.f);
+        else h_sc64.assign(nE, 0.0);
+    }
+    const double V = pr->cellVolume;
     for (int k = 0; k < nB; k++) {
-        int si = s->slot_of[bi[k]], sj = s->slot_of[bj[k]];
-        uint32_t type = pr->BONDTYPE ? (uint32_t)pr->BONDTYPE[k] : 0u;
-        uint32_t flags = (type << TYPE_SHIFT) | ((pr->fail && pr->fail[k] == 0) ? FAILED_BIT : 0u);
-        long long ei = entry_at(si, fillpos[si]++);
-        long long ej = entry_at(sj, fillpos[sj]++);
-        h_nbr[ei] = (uint32_t)sj | flags;
-        h_nbr[ej] = (uint32_t)si | flags;
+        const int si = s->slot_of[bi[k]], sj = s->slot_of[bj[k]];
+        const uint32_t type = pr->BONDTYPE ? (uint32_t)pr->BONDTYPE[k] : 0u;
+        const uint32_t flags = (type << TYPE_SHIFT) | ((pr->fail && pr->fail[k] == 0) ? FAILED_BIT : 0u);
+        const long long ei = entry_at(si, fillpos[si]++);
+        const long long ej = entry_at(sj, fillpos[sj]++);
         s->entry_i[k] = ei;
         s->entry_j[k] = ej;
+        const long long ends[2] = {ei, ej};
+        const int other[2] = {sj, si};
+        for (int t = 0; t < 2; t++) {
+            const long long e = ends[t];
+            uint4 r;
+            r.x = (uint32_t)other[t] | flags;
+            if (!mixed) {
+                uint32_t ti = 0;
+                if (cmode == PM_TABLE) ti |= cidx[k];
+                if (s0mode == PM_TABLE || scmode == PM_TABLE) ti |= pidx[k] << 16;
+                r.y = ti;
+                r.z = 0; // history starts at 0
+                r.w = 0;
+                if (cmode == PM_STREAM) h_c64[e] = pr->BONDSTIFFNESS[k];
+                if (s0mode == PM_STREAM) h_s064[e] = pr->s0[k];
+                if (scmode == PM_STREAM) h_sc64[e] = pr->sc[k];
+            } else {
+                float pin = 0.f;
+                if (s0mode == PM_INLINE) pin = (float)pr->s0[k];
+                else if (scmode == PM_INLINE) pin = (float)pr->sc[k];
+                uint32_t pb;
+                std::memcpy(&pb, &pin, 4);
+                r.y = pb;
+                r.z = 0;
+                const float kap = (float)(pr->BONDSTIFFNESS[k] * V * pr->VOLUMECORRECTIONFACTORS[k]);
+                std::memcpy(&r.w, &kap, 4);
+                if (s0mode == PM_STREAM) h_s032[e] = (float)pr->s0[k];
+                if (scmode == PM_STREAM) h_sc032[e] = (float)pr->sc[k];
+            }
+            h_rec[e] = r;
+        }
     }
-    // padding entries point at their own slot (never read: rows stop at rowlen)
+    // padding entries point at their own slot (their arithmetic is masked off)
     for (int sl = 0; sl < s->nSlices; sl++)
         for (int t = 0; t < SLICE; t++) {
             int slot = sl * SLICE + t;
             long long w = (slice_off[sl + 1] - slice_off[sl]) / SLICE;
-            for (long long kk = rowlen[slot]; kk < w; kk++) h_nbr[entry_at(slot, (int)kk)] = (uint32_t)slot;
+            for (long long kk = rowlen[slot]; kk < w; kk++) h_rec[entry_at(slot, (int)kk)].x = (uint32_t)slot;
         }
-
-    // ---- per-entry values
-    s->s0arr = pr->s0 != nullptr;
-    s->scarr = pr->sc != nullptr;
-    const double V = pr->cellVolume;
-    std::vector<double> h_hist64, h_coef64, h_s064, h_sc64;
-    std::vector<float> h_hist32, h_coef32, h_s032, h_sc32;
-    if (mixed) {
-        h_hist32.assign(nE, 0.f);
-        h_coef32.assign(nE, 0.f);
-        if (s->s0arr) h_s032.assign(nE, 0.f);
-        if (s->scarr) h_sc32.assign(nE, 0.f);
-    } else {
-        h_hist64.assign(nE, 0.0);
-        h_coef64.assign(nE, 0.0);
-        if (s->s0arr) h_s064.assign(nE, 0.0);
-        if (s->scarr) h_sc64.assign(nE, 0.0);
-    }
-    for (int k = 0; k < nB; k++) {
-        long long ends[2] = {s->entry_i[k], s->entry_j[k]};
-        for (long long e : ends) {
-            if (mixed) {
-                h_coef32[e] = (float)(pr->BONDSTIFFNESS[k] * V * pr->VOLUMECORRECTIONFACTORS[k]);
-                if (s->s0arr) h_s032[e] = (float)pr->s0[k];
-                if (s->scarr) h_sc32[e] = (float)pr->sc[k];
-            } else {
-                h_coef64[e] = pr->BONDSTIFFNESS[k];
-                if (s->s0arr) h_s064[e] = pr->s0[k];
-                if (s->scarr) h_sc64[e] = pr->sc[k];
-            }
-        }
-    }
 
     // ---- device allocations
     BBPD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -993,11 +1060,13 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.nN = nN;
     P.nSlots = nSlots;
     int rc = 0;
-    double *Xd, *pos[2], *ud = nullptr, *vd, *Fd, *Fpd = nullptr, *rhod, *massd = nullptr;
+    double *Xd, *ud = nullptr, *vd, *Fd, *Fpd = nullptr, *rhod, *massd = nullptr;
+    NodeRec *posd[2], *Xrecd = nullptr;
     unsigned char *consd;
     if ((rc = dalloc(&Xd, 3 * (size_t)nSlots, s->owned))) return fail(rc);
     for (int q = 0; q < 2; q++)
-        if ((rc = dalloc(&pos[q], 3 * (size_t)nSlots, s->owned))) return fail(rc);
+        if ((rc = dalloc(&posd[q], nSlots, s->owned))) return fail(rc);
+    if (!lattice && (rc = dalloc(&Xrecd, nSlots, s->owned))) return fail(rc);
     if (!mixed && (rc = dalloc(&ud, 3 * (size_t)nSlots, s->owned))) return fail(rc);
     if ((rc = dalloc(&vd, 3 * (size_t)nSlots, s->owned))) return fail(rc);
     if ((rc = dalloc(&Fd, 3 * (size_t)nSlots, s->owned))) return fail(rc);
@@ -1007,16 +1076,18 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     if ((rc = dalloc(&consd, nSlots, s->owned))) return fail(rc);
     int *rowlend;
     long long *soffd;
-    uint32_t *nbrd;
+    uint4 *recd;
     if ((rc = dalloc(&rowlend, nSlots, s->owned))) return fail(rc);
     if ((rc = dalloc(&soffd, s->nSlices + 1, s->owned))) return fail(rc);
-    if ((rc = dalloc(&nbrd, nE, s->owned))) return fail(rc);
-    void *histd, *coefd, *s0d = nullptr, *scd = nullptr;
+    if ((rc = dalloc(&recd, nE, s->owned))) return fail(rc);
+    double *cstrd = nullptr, *tabd = nullptr;
+    void *s0d = nullptr, *scd = nullptr;
     const size_t ew = mixed ? 4 : 8;
-    if ((rc = dalloc((char **)&histd, nE * ew, s->owned))) return fail(rc);
-    if ((rc = dalloc((char **)&coefd, nE * ew, s->owned))) return fail(rc);
-    if (s->s0arr && (rc = dalloc((char **)&s0d, nE * ew, s->owned))) return fail(rc);
-    if (s->scarr && (rc = dalloc((char **)&scd, nE * ew, s->owned))) return fail(rc);
+    if (!h_c64.empty() && (rc = dalloc(&cstrd, nE, s->owned))) return fail(rc);
+    if (s0mode == PM_STREAM && (rc = dalloc((char **)&s0d, nE * ew, s->owned))) return fail(rc);
+    if (scmode == PM_STREAM && (rc = dalloc((char **)&scd, nE * ew, s->owned))) return fail(rc);
+    const size_t ntab = tabc.size() + tabp.size();
+    if ((rc = dalloc(&tabd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
     double *partd;
     if ((rc = dalloc(&partd, 2 * (size_t)(nSlots / BLOCK) + 2, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->ds, 1, s->owned))) return fail(rc);
@@ -1025,24 +1096,42 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
 
     // ---- host staging in slot order
     {
-        std::vector<double> hX(3 * (size_t)nSlots, 0.0), hx(3 * (size_t)nSlots, 0.0), zero(3 * (size_t)nSlots, 0.0);
+        std::vector<double> hX(3 * (size_t)nSlots, 0.0), zero(3 * (size_t)nSlots, 0.0);
+        std::vector<NodeRec> hpos(nSlots), hXr(lattice ? 0 : nSlots);
+        std::memset(hpos.data(), 0, sizeof(NodeRec) * nSlots);
         std::vector<double> hrho(nSlots, 1.0), hm(nSlots, 1.0);
         std::vector<unsigned char> hc(nSlots, 0);
         for (int n = 0; n < nN; n++) {
             int q = s->slot_of[n];
+            double xv[3];
             for (int d = 0; d < 3; d++) {
-                hX[q + (size_t)nSlots * d] = X[n + (size_t)nN * d];
-                hx[q + (size_t)nSlots * d] = X[n + (size_t)nN * d];
+                xv[d] = X[n + (size_t)nN * d];
+                hX[q + (size_t)nSlots * d] = xv[d];
                 if (pr->CONSTRAINTFLAG && pr->CONSTRAINTFLAG[n + (size_t)nN * d] == 1) hc[q] |= (unsigned char)(1u << d);
+            }
+            // FP64 form: positions start at deformedCoordinates = undeformedCoordinates
+            // (dynamicsolverdisplacementcontrolled.m:33); mixed form: displacements start at 0
+            hpos[q].x = mixed ? 0.0 : xv[0];
+            hpos[q].y = mixed ? 0.0 : xv[1];
+            hpos[q].z = mixed ? 0.0 : xv[2];
+            if (lattice) {
+                uint64_t k0 = (uint64_t)std::floor(xv[0] / hlat[0] + 0.5);
+                uint64_t k1 = (uint64_t)std::floor(xv[1] / hlat[1] + 0.5);
+                uint64_t k2 = (uint64_t)std::floor(xv[2] / hlat[2] + 0.5);
+                hpos[q].q = k0 | (k1 << LAT_BITS) | (k2 << (2 * LAT_BITS));
+            } else {
+                hXr[q].x = xv[0];
+                hXr[q].y = xv[1];
+                hXr[q].z = xv[2];
+                hXr[q].q = 0;
             }
             hrho[q] = pr->DENSITY[n];
             if (pr->massVector) hm[q] = pr->massVector[n];
         }
         BBPD_HIP(hipMemcpy(Xd, hX.data(), sizeof(double) * hX.size(), hipMemcpyHostToDevice));
-        // FP64 form: positions start at deformedCoordinates = undeformedCoordinates
-        // (dynamicsolverdisplacementcontrolled.m:33); mixed form: displacements start at 0
         for (int q = 0; q < 2; q++)
-            BBPD_HIP(hipMemcpy(pos[q], mixed ? zero.data() : hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
+            BBPD_HIP(hipMemcpy(posd[q], hpos.data(), sizeof(NodeRec) * nSlots, hipMemcpyHostToDevice));
+        if (Xrecd) BBPD_HIP(hipMemcpy(Xrecd, hXr.data(), sizeof(NodeRec) * nSlots, hipMemcpyHostToDevice));
         if (ud) BBPD_HIP(hipMemcpy(ud, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(vd, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(Fd, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
@@ -1053,17 +1142,14 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     }
     BBPD_HIP(hipMemcpy(rowlend, rowlen.data(), sizeof(int) * nSlots, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy(soffd, slice_off.data(), sizeof(long long) * slice_off.size(), hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy(nbrd, h_nbr.data(), sizeof(uint32_t) * nE, hipMemcpyHostToDevice));
-    if (mixed) {
-        BBPD_HIP(hipMemcpy(histd, h_hist32.data(), 4 * nE, hipMemcpyHostToDevice));
-        BBPD_HIP(hipMemcpy(coefd, h_coef32.data(), 4 * nE, hipMemcpyHostToDevice));
-        if (s0d) BBPD_HIP(hipMemcpy(s0d, h_s032.data(), 4 * nE, hipMemcpyHostToDevice));
-        if (scd) BBPD_HIP(hipMemcpy(scd, h_sc32.data(), 4 * nE, hipMemcpyHostToDevice));
-    } else {
-        BBPD_HIP(hipMemcpy(histd, h_hist64.data(), 8 * nE, hipMemcpyHostToDevice));
-        BBPD_HIP(hipMemcpy(coefd, h_coef64.data(), 8 * nE, hipMemcpyHostToDevice));
-        if (s0d) BBPD_HIP(hipMemcpy(s0d, h_s064.data(), 8 * nE, hipMemcpyHostToDevice));
-        if (scd) BBPD_HIP(hipMemcpy(scd, h_sc64.data(), 8 * nE, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy(recd, h_rec.data(), sizeof(uint4) * nE, hipMemcpyHostToDevice));
+    if (cstrd) BBPD_HIP(hipMemcpy(cstrd, h_c64.data(), 8 * nE, hipMemcpyHostToDevice));
+    if (s0d) BBPD_HIP(hipMemcpy(s0d, mixed ? (void *)h_s032.data() : (void *)h_s064.data(), ew * nE, hipMemcpyHostToDevice));
+    if (scd) BBPD_HIP(hipMemcpy(scd, mixed ? (void *)h_sc032.data() : (void *)h_sc64.data(), ew * nE, hipMemcpyHostToDevice));
+    {
+        std::vector<double> t(tabc);
+        t.insert(t.end(), tabp.begin(), tabp.end());
+        if (!t.empty()) BBPD_HIP(hipMemcpy(tabd, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
     }
     DevScalars zs{};
     BBPD_HIP(hipMemcpy(s->ds, &zs, sizeof(zs), hipMemcpyHostToDevice));
@@ -1073,11 +1159,12 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.X0 = comp(Xd, 0);
     P.X1 = comp(Xd, 1);
     P.X2 = comp(Xd, 2);
-    for (int q = 0; q < 2; q++) {
-        P.p0[q] = comp(pos[q], 0);
-        P.p1[q] = comp(pos[q], 1);
-        P.p2[q] = comp(pos[q], 2);
-    }
+    P.pos[0] = posd[0];
+    P.pos[1] = posd[1];
+    P.Xrec = Xrecd;
+    P.hx = hlat[0];
+    P.hy = hlat[1];
+    P.hz = hlat[2];
     P.u0 = comp(ud, 0);
     P.u1 = comp(ud, 1);
     P.u2 = comp(ud, 2);
@@ -1095,11 +1182,17 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.cons = consd;
     P.rowlen = rowlend;
     P.slice_off = soffd;
-    P.nbr = nbrd;
-    P.hist = histd;
-    P.coef = coefd;
-    P.s0e = s0d;
-    P.sce = scd;
+    P.rec = recd;
+    P.cstr = cstrd;
+    P.s0str = s0d;
+    P.scstr = scd;
+    P.tab = tabd;
+    P.ntabc = (int)tabc.size();
+    P.ntabp = (int)tabp.size();
+    P.cmode = cmode;
+    P.s0mode = s0mode;
+    P.scmode = scmode;
+    P.adr = pr->integrator == BBPD_INTEGRATOR_ADR ? 1 : 0;
     P.V = V;
     P.horizon = horizon;
     P.hmr = hmr;
@@ -1117,6 +1210,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.D = pr->DAMPING;
     P.partials = partd;
     P.ds = s->ds;
+    s->lds_bytes = mixed ? 0 : sizeof(double) * ntab;
 
     // ---- contact bodies and observables
     StepEnd &S = s->S;
@@ -1164,10 +1258,9 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     S.cmod2 = slot_or_none(pr->cmodNode2);
     S.hist = s->hist;
 
-    s->device_bytes = 0;
-    s->device_bytes += sizeof(double) * 3 * (size_t)nSlots * (mixed ? 5 : 6) + sizeof(double) * 2 * nSlots + nSlots;
-    s->device_bytes += (4 + 2 * ew + (s->s0arr ? ew : 0) + (s->scarr ? ew : 0)) * (size_t)nE;
-    if (Fpd) s->device_bytes += sizeof(double) * 3 * (size_t)nSlots;
+    s->device_bytes = sizeof(NodeRec) * (size_t)nSlots * (lattice ? 2 : 3) +
+                      sizeof(double) * (size_t)nSlots * (3 * (mixed ? 3 : 4) + 2 + (Fpd ? 3 : 0)) + nSlots +
+                      sizeof(uint4) * (size_t)nE + (cstrd ? 8 * nE : 0) + (s0d ? ew * nE : 0) + (scd ? ew * nE : 0);
 
     s->ev.resize(2048);
     for (auto &e : s->ev) BBPD_HIP(hipEventCreate(&e));
@@ -1274,39 +1367,100 @@ extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
     return BBPD_OK;
 }
 
+namespace {
+
+// device nodal state <-> reference layout (nNodes x 3, column-major) on the host
+int fetch_soa(bbpd_solver *s, double *const *comps, double *dst)
+{
+    const int nN = s->nN, nS = s->nSlots;
+    std::vector<double> buf((size_t)nS * 3);
+    for (int dd = 0; dd < 3; dd++)
+        BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, comps[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nN; n++)
+        for (int dd = 0; dd < 3; dd++) dst[n + (size_t)nN * dd] = buf[s->slot_of[n] + (size_t)nS * dd];
+    return BBPD_OK;
+}
+
+int put_soa(bbpd_solver *s, double *const *comps, const double *src)
+{
+    const int nN = s->nN, nS = s->nSlots;
+    std::vector<double> buf((size_t)nS * 3);
+    for (int dd = 0; dd < 3; dd++)
+        BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, comps[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nN; n++)
+        for (int dd = 0; dd < 3; dd++) buf[s->slot_of[n] + (size_t)nS * dd] = src[n + (size_t)nN * dd];
+    for (int dd = 0; dd < 3; dd++)
+        BBPD_HIP(hipMemcpy(comps[dd], buf.data() + (size_t)nS * dd, sizeof(double) * nS, hipMemcpyHostToDevice));
+    return BBPD_OK;
+}
+
+int fetch_pos(bbpd_solver *s, int buf_id, double *dst)
+{
+    const int nN = s->nN, nS = s->nSlots;
+    std::vector<NodeRec> r(nS);
+    BBPD_HIP(hipMemcpy(r.data(), s->P.pos[buf_id], sizeof(NodeRec) * nS, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nN; n++) {
+        const NodeRec &q = r[s->slot_of[n]];
+        dst[n] = q.x;
+        dst[n + (size_t)nN] = q.y;
+        dst[n + 2 * (size_t)nN] = q.z;
+    }
+    return BBPD_OK;
+}
+
+int put_pos(bbpd_solver *s, int buf_id, const double *src)
+{
+    const int nN = s->nN, nS = s->nSlots;
+    std::vector<NodeRec> r(nS);
+    BBPD_HIP(hipMemcpy(r.data(), s->P.pos[buf_id], sizeof(NodeRec) * nS, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nN; n++) {
+        NodeRec &q = r[s->slot_of[n]];
+        q.x = src[n];
+        q.y = src[n + (size_t)nN];
+        q.z = src[n + 2 * (size_t)nN];
+    }
+    BBPD_HIP(hipMemcpy(s->P.pos[buf_id], r.data(), sizeof(NodeRec) * nS, hipMemcpyHostToDevice));
+    return BBPD_OK;
+}
+
+long long current_step(bbpd_solver *s, int *rc)
+{
+    DevScalars d;
+    hipError_t e = hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        *rc = bbpd::hip_fail(e, "hipMemcpy (step counter)");
+        return 0;
+    }
+    *rc = BBPD_OK;
+    return d.step;
+}
+
+} // namespace
+
 extern "C" int bbpd_solver_get_nodal(bbpd_solver *s, double *u, double *v, double *F, double *x)
 {
     BBPD_REQUIRE(s, "null solver");
     BBPD_HIP(hipSetDevice(s->device));
     BBPD_HIP(hipStreamSynchronize(s->stream));
-    DevScalars d;
-    BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
-    const int cur = (int)(d.step & 1);
+    int rc = 0;
+    const long long step = current_step(s, &rc);
+    if (rc) return rc;
+    const int cur = (int)(step & 1);
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
-    const int nN = s->nN, nS = s->nSlots;
-    std::vector<double> buf((size_t)nS * 3), Xb;
-    auto fetch = [&](double *const *comps, double *dst) -> int {
-        for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, comps[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
-        for (int n = 0; n < nN; n++)
-            for (int dd = 0; dd < 3; dd++) dst[n + (size_t)nN * dd] = buf[s->slot_of[n] + (size_t)nS * dd];
-        return BBPD_OK;
-    };
-    double *pc[3] = {s->P.p0[cur], s->P.p1[cur], s->P.p2[cur]};
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
     double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
     double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
-    int rc;
-    if (u && (rc = fetch(mixed ? pc : uc, u))) return rc;
-    if (v && (rc = fetch(vc, v))) return rc;
-    if (F && (rc = fetch(fc, F))) return rc;
+    if (u && (rc = mixed ? fetch_pos(s, cur, u) : fetch_soa(s, uc, u))) return rc;
+    if (v && (rc = fetch_soa(s, vc, v))) return rc;
+    if (F && (rc = fetch_soa(s, fc, F))) return rc;
     if (x) {
         if (!mixed) {
-            if ((rc = fetch(pc, x))) return rc;
+            if ((rc = fetch_pos(s, cur, x))) return rc;
         } else {
-            std::vector<double> uu((size_t)nN * 3), XX((size_t)nN * 3);
+            std::vector<double> uu((size_t)s->nN * 3), XX((size_t)s->nN * 3);
             double *Xc[3] = {(double *)s->P.X0, (double *)s->P.X1, (double *)s->P.X2};
-            if ((rc = fetch(pc, uu.data()))) return rc;
-            if ((rc = fetch(Xc, XX.data()))) return rc;
+            if ((rc = fetch_pos(s, cur, uu.data()))) return rc;
+            if ((rc = fetch_soa(s, Xc, XX.data()))) return rc;
             for (size_t q = 0; q < uu.size(); q++) x[q] = XX[q] + uu[q];
         }
     }
@@ -1320,61 +1474,31 @@ extern "C" int bbpd_solver_get_bonds(bbpd_solver *s, double *H, double *flag, do
     BBPD_HIP(hipStreamSynchronize(s->stream));
     const long long nE = s->nPadded;
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
-    std::vector<uint32_t> w(nE);
-    BBPD_HIP(hipMemcpy(w.data(), s->P.nbr, sizeof(uint32_t) * nE, hipMemcpyDeviceToHost));
-    std::vector<double> h64;
-    std::vector<float> h32;
-    if (H) {
+    std::vector<uint4> r(nE);
+    BBPD_HIP(hipMemcpy(r.data(), s->P.rec, sizeof(uint4) * nE, hipMemcpyDeviceToHost));
+    auto hist_of = [&](long long e) {
         if (mixed) {
-            h32.resize(nE);
-            BBPD_HIP(hipMemcpy(h32.data(), s->P.hist, 4 * nE, hipMemcpyDeviceToHost));
-        } else {
-            h64.resize(nE);
-            BBPD_HIP(hipMemcpy(h64.data(), s->P.hist, 8 * nE, hipMemcpyDeviceToHost));
+            float f;
+            std::memcpy(&f, &r[e].z, 4);
+            return (double)f;
         }
-    }
+        return bits_to_double(r[e].z, r[e].w);
+    };
     for (int k = 0; k < s->nB; k++) {
-        long long e = s->entry_i[k], e2 = s->entry_j[k];
+        const long long e = s->entry_i[k], e2 = s->entry_j[k];
         if (H) {
-            double a = mixed ? (double)h32[e] : h64[e];
-            double b = mixed ? (double)h32[e2] : h64[e2];
+            const double a = hist_of(e), b = hist_of(e2);
             if (!bits_equal(a, b)) {
                 bbpd::set_error("internal: the two ends of bond " + std::to_string(k + 1) + " disagree");
                 return BBPD_ERR_UNSUPPORTED;
             }
             H[k] = a;
         }
-        if (flag) flag[k] = (w[e] & FLAG_BIT) ? 1.0 : 0.0;
-        if (failout) failout[k] = (w[e] & FAILED_BIT) ? 0.0 : 1.0;
+        if (flag) flag[k] = (r[e].x & FLAG_BIT) ? 1.0 : 0.0;
+        if (failout) failout[k] = (r[e].x & FAILED_BIT) ? 0.0 : 1.0;
     }
     return BBPD_OK;
 }
-
-namespace {
-
-// nodal array in reference layout (nNodes x 3, column-major) <-> three slot-ordered device arrays
-int put_nodal(bbpd_solver *s, double *const *dst, const double *src)
-{
-    const int nN = s->nN, nS = s->nSlots;
-    std::vector<double> buf((size_t)nS * 3, 0.0);
-    for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, dst[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
-    for (int n = 0; n < nN; n++)
-        for (int dd = 0; dd < 3; dd++) buf[s->slot_of[n] + (size_t)nS * dd] = src[n + (size_t)nN * dd];
-    for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(dst[dd], buf.data() + (size_t)nS * dd, sizeof(double) * nS, hipMemcpyHostToDevice));
-    return BBPD_OK;
-}
-
-int get_nodal3(bbpd_solver *s, double *const *srcd, double *dst)
-{
-    const int nN = s->nN, nS = s->nSlots;
-    std::vector<double> buf((size_t)nS * 3);
-    for (int dd = 0; dd < 3; dd++) BBPD_HIP(hipMemcpy(buf.data() + (size_t)nS * dd, srcd[dd], sizeof(double) * nS, hipMemcpyDeviceToHost));
-    for (int n = 0; n < nN; n++)
-        for (int dd = 0; dd < 3; dd++) dst[n + (size_t)nN * dd] = buf[s->slot_of[n] + (size_t)nS * dd];
-    return BBPD_OK;
-}
-
-} // namespace
 
 extern "C" int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, double *v, double *x, double *F,
                                       double *Fp, double *H, double *flag, double *fail)
@@ -1382,15 +1506,15 @@ extern "C" int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, 
     BBPD_REQUIRE(s, "null solver");
     BBPD_HIP(hipSetDevice(s->device));
     BBPD_HIP(hipStreamSynchronize(s->stream));
-    DevScalars d;
-    BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
-    if (step) *step = d.step;
-    int rc = bbpd_solver_get_nodal(s, u, v, F, x);
+    int rc = 0;
+    const long long st = current_step(s, &rc);
     if (rc) return rc;
+    if (step) *step = st;
+    if ((rc = bbpd_solver_get_nodal(s, u, v, F, x))) return rc;
     if (Fp) {
         if (s->P.Fp0) {
             double *fp[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
-            if ((rc = get_nodal3(s, fp, Fp))) return rc;
+            if ((rc = fetch_soa(s, fp, Fp))) return rc;
         } else {
             std::memset(Fp, 0, sizeof(double) * 3 * (size_t)s->nN);
         }
@@ -1407,46 +1531,41 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
     BBPD_HIP(hipStreamSynchronize(s->stream));
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     const int cur = (int)(step & 1);
-    double *pc[3] = {s->P.p0[cur], s->P.p1[cur], s->P.p2[cur]};
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
     double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
     double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
     double *fpc[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
     int rc;
-    if (u && (rc = put_nodal(s, mixed ? pc : uc, u))) return rc;
-    if (!mixed && x && (rc = put_nodal(s, pc, x))) return rc;
-    if (v && (rc = put_nodal(s, vc, v))) return rc;
-    if (F && (rc = put_nodal(s, fc, F))) return rc;
-    if (Fp && s->P.Fp0 && (rc = put_nodal(s, fpc, Fp))) return rc;
+    if (u && (rc = mixed ? put_pos(s, cur, u) : put_soa(s, uc, u))) return rc;
+    if (!mixed && x && (rc = put_pos(s, cur, x))) return rc;
+    if (v && (rc = put_soa(s, vc, v))) return rc;
+    if (F && (rc = put_soa(s, fc, F))) return rc;
+    if (Fp && s->P.Fp0 && (rc = put_soa(s, fpc, Fp))) return rc;
     const long long nE = s->nPadded;
-    if (H) {
+    if (H)
         for (int k = 0; k < s->nB; k++)
             if (!(H[k] >= 0 && H[k] <= 1)) {
                 bbpd::set_error("bondSofteningFactor must lie in [0, 1]");
                 return BBPD_ERR_ARG;
             }
-        if (mixed) {
-            std::vector<float> h(nE);
-            BBPD_HIP(hipMemcpy(h.data(), s->P.hist, 4 * nE, hipMemcpyDeviceToHost));
-            for (int k = 0; k < s->nB; k++) h[s->entry_i[k]] = h[s->entry_j[k]] = (float)H[k];
-            BBPD_HIP(hipMemcpy(s->P.hist, h.data(), 4 * nE, hipMemcpyHostToDevice));
-        } else {
-            std::vector<double> h(nE);
-            BBPD_HIP(hipMemcpy(h.data(), s->P.hist, 8 * nE, hipMemcpyDeviceToHost));
-            for (int k = 0; k < s->nB; k++) h[s->entry_i[k]] = h[s->entry_j[k]] = H[k];
-            BBPD_HIP(hipMemcpy(s->P.hist, h.data(), 8 * nE, hipMemcpyHostToDevice));
-        }
-    }
-    if (flag || fail) {
-        std::vector<uint32_t> w(nE);
-        BBPD_HIP(hipMemcpy(w.data(), s->P.nbr, sizeof(uint32_t) * nE, hipMemcpyDeviceToHost));
+    if (H || flag || fail) {
+        std::vector<uint4> r(nE);
+        BBPD_HIP(hipMemcpy(r.data(), s->P.rec, sizeof(uint4) * nE, hipMemcpyDeviceToHost));
         for (int k = 0; k < s->nB; k++) {
             for (long long e : {s->entry_i[k], s->entry_j[k]}) {
-                if (flag) w[e] = (w[e] & ~FLAG_BIT) | (flag[k] == 1 ? FLAG_BIT : 0u);
-                if (fail) w[e] = (w[e] & ~FAILED_BIT) | (fail[k] == 0 ? FAILED_BIT : 0u);
+                if (H) {
+                    if (mixed) {
+                        const float f = (float)H[k];
+                        std::memcpy(&r[e].z, &f, 4);
+                    } else {
+                        double_to_bits(H[k], r[e].z, r[e].w);
+                    }
+                }
+                if (flag) r[e].x = (r[e].x & ~FLAG_BIT) | (flag[k] == 1 ? FLAG_BIT : 0u);
+                if (fail) r[e].x = (r[e].x & ~FAILED_BIT) | (fail[k] == 0 ? FAILED_BIT : 0u);
             }
         }
-        BBPD_HIP(hipMemcpy(s->P.nbr, w.data(), sizeof(uint32_t) * nE, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->P.rec, r.data(), sizeof(uint4) * nE, hipMemcpyHostToDevice));
     }
     DevScalars d;
     BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
@@ -1461,15 +1580,15 @@ extern "C" int bbpd_solver_history(bbpd_solver *s, double *reaction, double *def
     BBPD_REQUIRE(s && n_out && cap >= 0, "bad arguments");
     BBPD_HIP(hipSetDevice(s->device));
     BBPD_HIP(hipStreamSynchronize(s->stream));
-    DevScalars d;
-    BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
-    long long from = std::max(s->hist_read, d.step - (long long)HIST_CAP);
-    long long cnt = std::min((long long)cap, d.step - from);
-    std::vector<double> h((size_t)N_HIST);
+    int rc = 0;
+    const long long step = current_step(s, &rc);
+    if (rc) return rc;
+    long long from = std::max(s->hist_read, step - (long long)HIST_CAP);
+    long long cnt = std::min((long long)cap, step - from);
+    std::vector<double> all((size_t)HIST_CAP * N_HIST);
+    if (cnt > 0) BBPD_HIP(hipMemcpy(all.data(), s->hist, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
     for (long long q = 0; q < cnt; q++) {
-        long long st = from + q;
-        BBPD_HIP(hipMemcpy(h.data(), s->hist + (size_t)(st % HIST_CAP) * N_HIST, sizeof(double) * N_HIST,
-                           hipMemcpyDeviceToHost));
+        const double *h = all.data() + (size_t)((from + q) % HIST_CAP) * N_HIST;
         if (reaction) reaction[q] = h[0];
         if (deflection) deflection[q] = h[2];
     }
@@ -1484,15 +1603,18 @@ extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *
     BBPD_REQUIRE(s, "null solver");
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     const int64_t w = mixed ? 4 : 8;
-    // algorithmic bytes of one bond-kernel launch: per entry the index word, the history, the
-    // coefficient and the per-bond law parameters; per node the state the kernel reads and writes
-    int64_t per_entry = 4 + w + w + (s->s0arr ? w : 0) + (s->scarr ? w : 0);
+    // algorithmic bytes of one bond-kernel launch as laid out here: per entry the 16-byte record
+    // plus any streamed per-bond quantity; per node the state the kernel reads and writes
+    int64_t per_entry = 16 + (s->P.cmode == PM_STREAM && !mixed ? 8 : 0) + (s->P.s0mode == PM_STREAM ? w : 0) +
+                        (s->P.scmode == PM_STREAM ? w : 0);
     int64_t per_node;
     if (s->integrator == BBPD_INTEGRATOR_ADR)
-        per_node = 24 /*X*/ + 24 /*pos*/ + 24 /*F w*/ + 24 /*Fp*/ + 24 /*v*/ + 24 /*u*/ + 8 /*m*/ + 4 /*rowlen*/;
+        per_node = 32 /*pos r*/ + 24 /*X*/ + 24 /*F w*/ + 24 /*Fp*/ + 24 /*v*/ + (mixed ? 0 : 24) /*u*/ + 8 /*m*/ +
+                   4 /*rowlen*/;
     else
-        per_node = 24 /*X*/ + 24 /*pos r*/ + 24 /*pos w*/ + 48 /*v rw*/ + (mixed ? 0 : 48) /*u rw*/ + 24 /*F w*/ +
+        per_node = 32 /*pos r*/ + 24 /*X*/ + 24 /*pos w*/ + 48 /*v rw*/ + (mixed ? 0 : 48) /*u rw*/ + 24 /*F w*/ +
                    8 /*rho*/ + 1 /*cons*/ + 4 /*rowlen*/;
+    if (!s->lattice) per_node += 32;
     if (device_bytes) *device_bytes = (int64_t)s->device_bytes;
     if (bond_bytes) *bond_bytes = per_entry * s->nEntries + per_node * s->nN;
     if (n_entries) *n_entries = s->nEntries;

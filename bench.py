@@ -83,9 +83,22 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
+                    help="which precision forms to run (profiling runs one)")
     ap.add_argument("--small", action="store_true", help="dx = 2.5 mm notched trilinear beam instead")
+    ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
+                    help="only copy a device buffer of this many GiB 5 times (scales HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy_gib > 0:
+        import torch
+        n = int(args.calibrate_copy_gib * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one GPU per process: --gpus must be 1 "
                                    "(the multi-GPU slab decomposition is a design, not built)"}))
@@ -94,15 +107,19 @@ def main():
     t0 = time.time()
     case = beam.case_trilinear_dx2p5() if args.small else beam.case_exponential_dx1p25()
     log(f"built {case.name}: {case.nNodes} nodes {case.nBonds} bonds in {time.time() - t0:.1f}s")
-    fp64 = run_form(case, _lib.FP64, args.steps, args.warmup)
-    log(f"fp64: {fp64['ms_per_step']:.3f} ms/step, kernel {fp64['kernel_ms']:.3f} ms, "
-        f"{fp64['achieved_gbs']:.0f} GB/s")
-    if args.fp64_only:
-        mixed = fp64
-    else:
+    fp64 = mixed = None
+    if args.form in ("both", "fp64"):
+        fp64 = run_form(case, _lib.FP64, args.steps, args.warmup)
+        log(f"fp64: {fp64['ms_per_step']:.3f} ms/step, kernel {fp64['kernel_ms']:.3f} ms, "
+            f"{fp64['achieved_gbs']:.0f} GB/s")
+    if args.form in ("both", "mixed"):
         mixed = run_form(case, _lib.MIXED, args.steps, args.warmup)
         log(f"mixed: {mixed['ms_per_step']:.3f} ms/step, kernel {mixed['kernel_ms']:.3f} ms, "
             f"{mixed['achieved_gbs']:.0f} GB/s")
+    if args.form != "both":
+        print(json.dumps({"form": args.form, "result": {k: v for k, v in (fp64 or mixed).items()
+                                                         if k not in ("obs", "info")}}), flush=True)
+        return
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(case)
@@ -119,7 +136,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 bond arithmetic, f64 state" if not args.fp64_only else "f64",
+        "dtype": "f32 bond arithmetic, f64 state",
         "data": "synthetic beam lattice generated in-process",
         "config": {
             "workload": case.name,
@@ -135,7 +152,7 @@ def main():
         "mixed_over_fp64": head["bond_updates_per_s"] / fp64["bond_updates_per_s"],
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["frac"], "traffic": None,
-                     "kernel": "k_bond_mixed" if not args.fp64_only else "k_bond_fp64",
+                     "kernel": "k_bond_mixed",
                      "kernel_ms": head["kernel_ms"], "bytes_per_launch": head["bytes_per_launch"]},
         "cpu_baseline": cpu,
     }
