@@ -1585,12 +1585,19 @@ extern "C" int bbpd_solver_history(bbpd_solver *s, double *reaction, double *def
     if (rc) return rc;
     long long from = std::max(s->hist_read, step - (long long)HIST_CAP);
     long long cnt = std::min((long long)cap, step - from);
-    std::vector<double> all((size_t)HIST_CAP * N_HIST);
-    if (cnt > 0) BBPD_HIP(hipMemcpy(all.data(), s->hist, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+    // copy only the ring rows [from, from + cnt), in at most two pieces
+    std::vector<double> rows((size_t)std::max(cnt, 0LL) * N_HIST);
+    long long done = 0;
+    while (done < cnt) {
+        const long long r0 = (from + done) % HIST_CAP;
+        const long long piece = std::min(cnt - done, (long long)HIST_CAP - r0);
+        BBPD_HIP(hipMemcpy(rows.data() + (size_t)done * N_HIST, s->hist + (size_t)r0 * N_HIST,
+                           sizeof(double) * N_HIST * (size_t)piece, hipMemcpyDeviceToHost));
+        done += piece;
+    }
     for (long long q = 0; q < cnt; q++) {
-        const double *h = all.data() + (size_t)((from + q) % HIST_CAP) * N_HIST;
-        if (reaction) reaction[q] = h[0];
-        if (deflection) deflection[q] = h[2];
+        if (reaction) reaction[q] = rows[(size_t)q * N_HIST];
+        if (deflection) deflection[q] = rows[(size_t)q * N_HIST + 2];
     }
     s->hist_read = from + cnt;
     *n_out = (int)cnt;

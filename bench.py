@@ -3,22 +3,32 @@
 Workload: the mixed-mode three-point-bend beam 500x100x50 mm at dx = 1.25 mm (400x40x80 =
 1,280,000 nodes, 74.5M bonds, offset notch, exponential softening), ADR with the penetrator
 staircase of 5e-6 m per 1000 iterations; one step = stretch -> law -> force -> nodal gather ->
-ADR update -> contact.  Both forms run: FP32-mixed (headline `value`) and plain FP64.
+ADR update -> contact.  Both forms run on the same steps: FP32-mixed (headline `value`) and plain
+FP64.
 
 Reported (one JSON line on stdout):
   value            bond updates/s of the FP32-mixed form = nBonds * K / seconds (inputs resident)
   fp64             the same for the plain FP64 form
-  roofline         dominant kernel (bond kernel, mixed form): algorithmic bytes per launch / average
-                   launch time from HIP events on the solver's stream, against 8 TB/s
+  roofline         dominant kernel (bond kernel, mixed form): the byte model of the hot-path survey,
+                   E*(4 + w_state + w_kappa + n_par*w) + N*180 with E = 2*nBonds entries, divided by
+                   the kernel's average launch time from HIP events on the solver's stream, against
+                   8 TB/s; `traffic` is the kernel's measured HBM bytes per launch (rocprofv3
+                   FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for a device
+                   copy of known size)
   cpu_baseline     the oracle's OpenMP restatement of the reference CPU passes timed on the host on
                    a bounded number of steps of the same workload
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -33,14 +43,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def survey_model_bytes(case, mixed: bool) -> int:
+    """Bytes one bond-kernel launch must move in the node-centric layout with kappa stored per
+    entry: E*(4 + w_state + w_kappa + n_par*w) + N*180."""
+    w = 4 if mixed else 8
+    n_par = int(isinstance(case.s0, np.ndarray)) + int(isinstance(case.sc, np.ndarray))
+    E = 2 * case.nBonds
+    return E * (4 + w + w + n_par * w) + case.nNodes * 180
+
+
 def run_form(case, precision, steps, warmup):
     import torch
+    from bb_pd_amd import _lib
     from bb_pd_amd.solver import Solver
     t0 = time.time()
     sol = Solver(case, precision)
     log(f"  setup {time.time() - t0:.1f}s, info {sol.info()}")
     sol.step(warmup)
     sol.sync()
+    sol.history()  # discard the warm-up record
     sol.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -52,12 +73,14 @@ def run_form(case, precision, steps, warmup):
     sol.set_timing(False)
     info = sol.info()
     obs = sol.observables()
+    reaction, _ = sol.history()
     sol.close()
-    bytes_per_launch = info["bond_kernel_bytes"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    mixed = precision == _lib.MIXED
+    model = survey_model_bytes(case, mixed)
+    achieved = model / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return dict(seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=case.nBonds * steps / dt,
-                kernel_ms=avg_ms, launches=launches, bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
-                frac=achieved / HBM_PEAK_GBS, obs=obs, info=info)
+                kernel_ms=avg_ms, launches=launches, bytes_model=model, bytes_layout=info["bond_kernel_bytes"],
+                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, obs=obs, info=info, reaction=reaction)
 
 
 def cpu_baseline(case, target_s=20.0):
@@ -73,8 +96,47 @@ def cpu_baseline(case, target_s=20.0):
         n += 1
         t = time.perf_counter() - t0
     return dict(value=case.nBonds * n / t, unit="bond updates/s", cores=threads, kind="port",
-                sample=f"{n} steps of the same 74.5M-bond beam (Euler-Cromer order, exponential law, elastic "
-                       f"state), oracle/bbpd_oracle.c or_step_omp, {threads} OpenMP threads")
+                sample=f"{n} steps of the same {case.nBonds}-bond beam (passes A-E in the reference's "
+                       f"Euler-Cromer order, exponential law, elastic state), oracle/bbpd_oracle.c "
+                       f"or_step_omp with {threads} OpenMP threads")
+
+
+def _pmc(counter: str, args: list, kernel_match: str):
+    """Average value of one counter per dispatch of the matching kernel, from a rocprofv3 child run
+    of this script (the program itself follows `--`)."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), *args]
+        try:
+            subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=240)
+        except (subprocess.SubprocessError, OSError):
+            return None
+        vals = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    if kernel_match in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                        vals.append(float(r["Counter_Value"]))
+        return sum(vals) / len(vals) if vals else None
+
+
+def measure_traffic(form: str):
+    """HBM bytes per launch of the bond kernel: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by bytes / counter for a device copy of 1 GiB (read 1 GiB, write 1 GiB)."""
+    gib = 1 << 30
+    cal_args = ["--calibrate-copy-gib", "1"]
+    kargs = ["--form", form, "--steps", "5", "--warmup", "2", "--no-cpu-baseline", "--no-traffic"]
+    cf = _pmc("FETCH_SIZE", cal_args, "copyBuffer")
+    cw = _pmc("WRITE_SIZE", cal_args, "copyBuffer")
+    kf = _pmc("FETCH_SIZE", kargs, "k_bond")
+    kw = _pmc("WRITE_SIZE", kargs, "k_bond")
+    if None in (cf, cw, kf, kw) or cf <= 0 or cw <= 0:
+        return None
+    sf, sw = gib / cf, gib / cw
+    return dict(fetch_bytes=kf * sf, write_bytes=kw * sw, bytes=kf * sf + kw * sw, fetch_scale=sf, write_scale=sw)
 
 
 def main():
@@ -83,6 +145,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic measurement")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
                     help="which precision forms to run (profiling runs one)")
     ap.add_argument("--small", action="store_true", help="dx = 2.5 mm notched trilinear beam instead")
@@ -107,19 +170,29 @@ def main():
     t0 = time.time()
     case = beam.case_trilinear_dx2p5() if args.small else beam.case_exponential_dx1p25()
     log(f"built {case.name}: {case.nNodes} nodes {case.nBonds} bonds in {time.time() - t0:.1f}s")
+    # the rocprofv3 child runs go first, while this process has not touched the GPU
+    traffic = None if (args.no_traffic or args.form != "both") else measure_traffic("mixed")
+    if traffic:
+        log(f"traffic: {traffic['bytes'] / 1e9:.3f} GB per launch (fetch x{traffic['fetch_scale']:.2f}, "
+            f"write x{traffic['write_scale']:.2f})")
     fp64 = mixed = None
     if args.form in ("both", "fp64"):
         fp64 = run_form(case, _lib.FP64, args.steps, args.warmup)
         log(f"fp64: {fp64['ms_per_step']:.3f} ms/step, kernel {fp64['kernel_ms']:.3f} ms, "
-            f"{fp64['achieved_gbs']:.0f} GB/s")
+            f"{fp64['achieved_gbs']:.0f} GB/s (model bytes)")
     if args.form in ("both", "mixed"):
         mixed = run_form(case, _lib.MIXED, args.steps, args.warmup)
         log(f"mixed: {mixed['ms_per_step']:.3f} ms/step, kernel {mixed['kernel_ms']:.3f} ms, "
-            f"{mixed['achieved_gbs']:.0f} GB/s")
+            f"{mixed['achieved_gbs']:.0f} GB/s (model bytes)")
     if args.form != "both":
-        print(json.dumps({"form": args.form, "result": {k: v for k, v in (fp64 or mixed).items()
-                                                         if k not in ("obs", "info")}}), flush=True)
+        res = fp64 or mixed
+        print(json.dumps({"form": args.form, "result": {k: v for k, v in res.items()
+                                                         if k not in ("obs", "info", "reaction")}}), flush=True)
         return
+    # FP32-mixed against FP64 on the same steps: the load record (static reaction of the penetrator)
+    ra, rb = fp64["reaction"], mixed["reaction"]
+    peak = float(np.max(np.abs(ra))) if ra.size else 0.0
+    load_err = float(np.max(np.abs(ra - rb)) / peak) if peak > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(case)
@@ -147,13 +220,14 @@ def main():
             "state": f"steps {args.warmup + 1}..{args.warmup + args.steps} of the load history (elastic)",
         },
         "fp64": {"value": fp64["bond_updates_per_s"], "ms_per_step": fp64["ms_per_step"],
-                 "kernel_ms": fp64["kernel_ms"], "bytes_per_launch": fp64["bytes_per_launch"],
-                 "achieved_gbs": fp64["achieved_gbs"], "frac": fp64["frac"]},
+                 "kernel_ms": fp64["kernel_ms"], "bytes_model": fp64["bytes_model"],
+                 "bytes_layout": fp64["bytes_layout"], "achieved_gbs": fp64["achieved_gbs"], "frac": fp64["frac"]},
         "mixed_over_fp64": head["bond_updates_per_s"] / fp64["bond_updates_per_s"],
+        "mixed_vs_fp64_load_err": load_err,
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["frac"], "traffic": None,
-                     "kernel": "k_bond_mixed",
-                     "kernel_ms": head["kernel_ms"], "bytes_per_launch": head["bytes_per_launch"]},
+                     "frac": head["frac"], "traffic": traffic["bytes"] if traffic else None,
+                     "kernel": "k_bond (FP32-mixed)", "kernel_ms": head["kernel_ms"],
+                     "bytes_model": head["bytes_model"], "bytes_layout": head["bytes_layout"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
