@@ -1575,7 +1575,7 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
     return BBPD_OK;
 }
 
-extern "C" int bbpd_solver_history(bbpd_solver *s, double *reaction, double *deflection, int cap, int *n_out)
+extern "C" int bbpd_solver_history(bbpd_solver *s, double *out_rows, int cap, int *n_out)
 {
     BBPD_REQUIRE(s && n_out && cap >= 0, "bad arguments");
     BBPD_HIP(hipSetDevice(s->device));
@@ -1595,10 +1595,7 @@ extern "C" int bbpd_solver_history(bbpd_solver *s, double *reaction, double *def
                            sizeof(double) * N_HIST * (size_t)piece, hipMemcpyDeviceToHost));
         done += piece;
     }
-    for (long long q = 0; q < cnt; q++) {
-        if (reaction) reaction[q] = rows[(size_t)q * N_HIST];
-        if (deflection) deflection[q] = rows[(size_t)q * N_HIST + 2];
-    }
+    if (out_rows && cnt > 0) std::memcpy(out_rows, rows.data(), sizeof(double) * rows.size());
     s->hist_read = from + cnt;
     *n_out = (int)cnt;
     return BBPD_OK;

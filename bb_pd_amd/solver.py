@@ -107,14 +107,16 @@ class Solver:
         p = [ptr(k) for k in ("u", "v", "x", "F", "Fp", "H", "flag", "fail")]
         _lib.check(self._L.bbpd_solver_restore(self._h, int(step), *p), "bbpd_solver_restore")
 
-    def history(self, cap: int = 1 << 20):
-        """(reaction, deflection) of every step since the previous call."""
-        r = np.zeros(cap)
-        d = np.zeros(cap)
+    def history(self, cap: int = 1 << 20) -> dict:
+        """Per-step record since the previous call: arrays `reaction` (contact reaction of body 0),
+        `reaction_static` (sum of F_z * cellVolume over body 0's family), `deflection`, `cmod`."""
+        rows = np.zeros((cap, 4))
         n = C.c_int()
-        _lib.check(self._L.bbpd_solver_history(self._h, r.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p),
-                                               cap, C.byref(n)), "bbpd_solver_history")
-        return r[:n.value].copy(), d[:n.value].copy()
+        _lib.check(self._L.bbpd_solver_history(self._h, rows.ctypes.data_as(C.c_void_p), cap, C.byref(n)),
+                   "bbpd_solver_history")
+        r = rows[:n.value]
+        return dict(reaction=r[:, 0].copy(), reaction_static=r[:, 1].copy(), deflection=r[:, 2].copy(),
+                    cmod=r[:, 3].copy())
 
     def info(self) -> dict:
         v = [C.c_int64() for _ in range(4)]

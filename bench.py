@@ -73,7 +73,7 @@ def run_form(case, precision, steps, warmup):
     sol.set_timing(False)
     info = sol.info()
     obs = sol.observables()
-    reaction, _ = sol.history()
+    reaction = sol.history()["reaction_static"]
     sol.close()
     mixed = precision == _lib.MIXED
     model = survey_model_bytes(case, mixed)

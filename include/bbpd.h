@@ -231,8 +231,11 @@ int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, double *v, 
                            double *H, double *flag, double *fail);
 int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u, const double *v, const double *x,
                         const double *F, const double *Fp, const double *H, const double *flag, const double *fail);
-/* load-point reaction history: the body-0 reaction of every step since the last call */
-int bbpd_solver_history(bbpd_solver *s, double *reaction, double *deflection, int cap, int *n_out);
+/* Per-step record since the previous call, 4 doubles per step in this order: body-0 contact
+ * reaction (calculatecontactforce.m:121), static reaction (calculatereactionforceFast.m:3-5 over the
+ * body-0 family), deflection u_z(referenceNode), CMOD u_x(cmod2) - u_x(cmod1).  At most `cap` steps
+ * are returned (the device keeps the last 2^20). */
+int bbpd_solver_history(bbpd_solver *s, double *rows, int cap, int *n_out);
 /* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */
 int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel_bytes, int64_t *n_entries,
                      int64_t *n_padded_entries);

@@ -67,8 +67,17 @@ def test_fp64_euler_cromer_bit_exact(gpu, oracle, law):
             sol.step(100)
             run.step(100)
             compare_state(sol, run, exact=True)
-            r, d = sol.history()
-            assert same(r, run.reactions[-100:])
+            h = sol.history()
+            assert same(h["reaction"], run.reactions[-100:])
+            # the last row against the oracle's state: deflection and CMOD are plain reads, the
+            # static reaction a tree sum over the body-0 family (calculatereactionforceFast.m:3-5)
+            ref = case.referenceNode - 1
+            c1, c2 = (i - 1 for i in case.cmod)
+            assert same(h["deflection"][-1:], [run.u[ref, 2]])
+            assert same(h["cmod"][-1:], [run.u[c2, 0] - run.u[c1, 0]])
+            fam = case.bodies[0].family.astype(int) - 1
+            Fz = (run.F[fam, 2] * case.cellVolume).sum()
+            assert h["reaction_static"][-1] == pytest.approx(Fz, rel=1e-12, abs=1e-12 * np.abs(run.F).max())
         obs = sol.observables()
         assert obs["step"] == 300
         assert obs["n_softening"] == int(run.flag.sum())
@@ -226,8 +235,6 @@ def test_mixed_precision_error_report(gpu):
         ub, vb, Fb, xb = b.nodal()
         Ha, fla, _ = a.bonds()
         Hb, flb, _ = b.bonds()
-        ra, _ = a.history()
-        rb, _ = b.history()
         err_u = np.abs(ua - ub).max() / np.abs(ua).max()
         err_F = np.abs(Fa - Fb).max() / np.abs(Fa).max()
         flips = int((fla != flb).sum())
