@@ -55,6 +55,9 @@ constexpr int N_HIST = 4;      // reaction, static reaction, deflection, CMOD
 constexpr int TAB_MAX = 2048;  // entries of one LDS parameter table
 constexpr int LAT_BITS = 21;   // bits per packed lattice coordinate
 constexpr uint64_t LAT_MASK = (1ull << LAT_BITS) - 1;
+// node-flag bits 0-2: DOF d constrained; bit 3: member of a contact family (its DOFs are driven by
+// the rigid-body projection, so the ADR Rayleigh quotient leaves the node out)
+constexpr unsigned char CONTACT_BIT = 1u << 3;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -87,7 +90,7 @@ struct Params {
     double *F0, *F1, *F2;
     double *Fp0, *Fp1, *Fp2; // ADR previous force
     const double *rho, *mass;
-    const unsigned char *cons; // bit d set = DOF d constrained
+    const unsigned char *cons; // bit d set = DOF d constrained; CONTACT_BIT = contact-family member
     // entries
     const int *rowlen;
     const long long *slice_off;
@@ -385,6 +388,11 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             P.F0[n] = f0;
             P.F1[n] = f1;
             P.F2[n] = f2;
+        }
+        // the quotient runs over the nodes no contact body drives: a family member's velocity
+        // after a projection is the projection's correction, and its term would outweigh the
+        // structure's by orders of magnitude, pinning cn to 0 or the clamp
+        if (real && !(P.cons[n] & CONTACT_BIT)) {
             const double mi = P.mass[n];
             const double ff[3] = {f0, f1, f2};
             const double fp[3] = {P.Fp0[n], P.Fp1[n], P.Fp2[n]};
@@ -1246,6 +1254,13 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         S.cz[bdy] = pr->centreZ[bdy];
         S.R[bdy] = pr->radius[bdy];
         S.moves[bdy] = pr->moves[bdy];
+    }
+    if (pr->integrator == BBPD_INTEGRATOR_ADR && pr->nBodies > 0) {
+        std::vector<unsigned char> hc(nSlots);
+        BBPD_HIP(hipMemcpy(hc.data(), consd, nSlots, hipMemcpyDeviceToHost));
+        for (int n = 0; n < nN; n++)
+            if (in_family[n]) hc[s->slot_of[n]] |= CONTACT_BIT;
+        BBPD_HIP(hipMemcpy(consd, hc.data(), nSlots, hipMemcpyHostToDevice));
     }
     S.schedule = pr->schedule;
     S.nTimeSteps = pr->nTimeSteps;

@@ -260,12 +260,16 @@ void or_timeintegrationeulercromer(double *a, const double *F, double D, double 
 /* ------------------------------------------------------------------------------------------------
  * ADR damping coefficient -- solvers/functions/calculatedampingcoefficient.m:59-112 (NOD == 3)
  *    Sequential node-major sum, massVector(i,1) for every DOF (:65,:71,:77).
+ *    `skip` (NULL = none) leaves nodes out of both sums: the stepped solver passes the members of
+ *    the contact families, whose DOFs the rigid-body projection drives.
  * ---------------------------------------------------------------------------------------------- */
-double or_calculatedampingcoefficient(const double *F, const double *m, const double *Fp, double DT,
-                                      const double *vp, const double *u, int nNodes)
+double or_calculatedampingcoefficient_masked(const double *F, const double *m, const double *Fp, double DT,
+                                             const double *vp, const double *u, int nNodes,
+                                             const unsigned char *skip)
 {
     double cn = 0, cn1 = 0, cn2 = 0;
     for (int i = 0; i < nNodes; i++) {
+        if (skip && skip[i]) continue;
         for (int d = 0; d < 3; d++) {
             size_t q = i + (size_t)nNodes * d;
             if (vp[q] != 0)
@@ -284,6 +288,12 @@ double or_calculatedampingcoefficient(const double *F, const double *m, const do
     }
     if (cn > 2) cn = 1.9;
     return cn;
+}
+
+double or_calculatedampingcoefficient(const double *F, const double *m, const double *Fp, double DT,
+                                      const double *vp, const double *u, int nNodes)
+{
+    return or_calculatedampingcoefficient_masked(F, m, Fp, DT, vp, u, nNodes, NULL);
 }
 
 /* Same quantity with the two sums returned, for checking a GPU reduction (a different order). */
@@ -476,7 +486,16 @@ double or_step(or_state *S, int iTimeStep, double disp, double *fz_bodies)
     if (S->integrator == 0) {
         or_timeintegrationeulercromer(S->a, S->F, S->DAMPING, S->v, S->rho, S->cons, S->DT, S->u, S->x, S->X, nN, 3);
     } else {
-        double cn = or_calculatedampingcoefficient(S->F, S->mass, S->Fp, S->DT, S->v, S->u, nN);
+        /* The Rayleigh quotient is taken over the nodes no contact body drives.  A family member's
+         * velocity after a projection is the projection's correction, not a mode of the structure,
+         * and its term (u^2 dF / (m DT v)) outweighs the rest by many orders of magnitude, which pins
+         * cn to 0 or the 1.9 clamp.  Membership is fixed, so cn stays a continuous function of the
+         * state. */
+        unsigned char *driven = (unsigned char *)calloc((size_t)nN, 1);
+        for (int b = 0; b < S->nBodies; b++)
+            for (int f = 0; f < S->nFams[b]; f++) driven[(int)S->families[b][f] - 1] = 1;
+        double cn = or_calculatedampingcoefficient_masked(S->F, S->mass, S->Fp, S->DT, S->v, S->u, nN, driven);
+        free(driven);
         /* v holds the previous half-step velocity on entry and the new one on exit */
         or_timeintegrationADR(iTimeStep, S->DT, S->v, S->F, S->mass, cn, S->v, S->u, S->cons, nN);
         for (size_t q = 0; q < (size_t)nN * 3; q++) S->x[q] = S->X[q] + S->u[q];

@@ -143,6 +143,15 @@ def calculatedampingcoefficient(F, m, Fp, DT, vp, u) -> float:
     return f(_p(F), _p(m), _p(Fp), C.c_double(DT), _p(vp), _p(u), C.c_int(F.shape[0]))
 
 
+def calculatedampingcoefficient_masked(F, m, Fp, DT, vp, u, skip) -> float:
+    """The quotient over the nodes with skip == 0 (the stepped solver's ADR with contact)."""
+    f = lib().or_calculatedampingcoefficient_masked
+    f.restype = C.c_double
+    sk = np.ascontiguousarray(skip, dtype=np.uint8)
+    return f(_p(F), _p(m), _p(Fp), C.c_double(DT), _p(vp), _p(u), C.c_int(F.shape[0]),
+             sk.ctypes.data_as(C.c_void_p))
+
+
 def dampingsums(F, m, Fp, DT, vp, u):
     c1, c2 = C.c_double(), C.c_double()
     lib().or_dampingsums(_p(F), _p(m), _p(Fp), C.c_double(DT), _p(vp), _p(u), C.c_int(F.shape[0]), C.byref(c1),

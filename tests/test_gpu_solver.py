@@ -7,7 +7,8 @@ Bars
   * exponential law: exp() differs from glibc by ulps, so 1e-10 relative (north-star tolerance for
     nodal forces) with identical flags on these short runs;
   * ADR: the damping coefficient is a tree sum on the device and a sequential sum in the reference,
-    so 1e-10 relative on displacements, forces and the reaction;
+    so one step from identical state agrees to 1e-12; free-running, the converged state at the end of
+    each load increment agrees to the ADR convergence floor (1e-4);
   * FP32-mixed: stated error against FP64 on every result, flips of near-threshold bonds counted.
 """
 import numpy as np
@@ -144,19 +145,68 @@ def test_checkpoint_resume_bitwise(gpu, integrator):
         assert same(got[k], ref[k]), k
 
 
-def test_fp64_adr_tolerance(gpu, oracle):
-    case = beam.build_beam("t", 24, 6, 10, 5e-3, law=_lib.LAW_BILINEAR, s0=1e-4, s0_perturb=0.01, sc=2e-3,
-                           integrator=_lib.ADR, iters_per_increment=50, increment=-1e-5)
-    print(f"case: ADR bilinear, {case.nBonds} bonds", flush=True)
+def adr_case(ipi=400):
+    """Symmetric 25 x 6 x 10 beam on node-column supports under ADR, loaded in increments of 2 um per
+    `ipi` iterations.  Frictionless cylinders leave rigid x translation and rotation about z free,
+    so u_x is held on the midspan plane (and u_y on one of its lines), as CONSTRAINTFLAG does."""
+    DX = 5e-3
+    case = beam.build_beam("t", 25, 6, 10, DX, law=_lib.LAW_BILINEAR, s0=1e-4, s0_perturb=0.01, sc=2e-3,
+                           integrator=_lib.ADR, iters_per_increment=ipi, increment=-2e-6,
+                           supports_x=(3 * DX, 23 * DX), penetrator_x=13 * DX)
+    mid = np.isclose(case.X[:, 0], 13 * DX)
+    cons = np.zeros((case.nNodes, 3), order="F")
+    cons[mid, 0] = 1
+    cons[mid & np.isclose(case.X[:, 1], DX), 1] = 1
+    case.CONSTRAINTFLAG = cons
+    return case
+
+
+def test_fp64_adr_per_step(gpu, oracle):
+    """ADR: from the oracle's exact state, one device step (force, damping coefficient as a tree sum
+    over the nodes no contact body drives, leapfrog update, constraints, contact) matches the
+    oracle's step within 1e-12; the damping coefficient itself within 1e-12."""
+    case = adr_case()
+    print(f"case: ADR bilinear per step, {case.nBonds} bonds", flush=True)
     run = oracle.OracleRun(case)
+    driven = np.zeros(case.nNodes, np.uint8)
+    for b in case.bodies:
+        driven[b.family.astype(int) - 1] = 1
     with Solver(case, _lib.FP64) as sol:
-        sol.step(200)
-        run.step(200)
-        compare_state(sol, run, exact=False, rtol=1e-10)
+        for k in (0, 1, 40, 399, 400, 401, 700):
+            run.step(k - run.step_count)
+            st = oracle_state(run)
+            sol.restore(k, **st)
+            sol.step(1)
+            run.step(1)
+            compare_state(sol, run, exact=False, rtol=1e-12)
+            cn = oracle.calculatedampingcoefficient_masked(run.F, case.mass, st["F"], case.DT, st["v"], st["u"], driven)
+            got = sol.observables()["cn"]
+            assert got == pytest.approx(cn, rel=1e-12, abs=1e-14), (k, got, cn)
+
+
+def test_fp64_adr_increments(gpu, oracle):
+    """ADR free-running over three converged load increments: the static reaction, the deflection and
+    the softening flags at the end of each increment.  Near convergence the damping estimate is
+    noise-driven (velocities ~1e-11), so trajectories that differ by rounding agree to the ADR
+    convergence floor, not bit for bit: 1e-4 relative."""
+    ipi = 400
+    case = adr_case(ipi)
+    print(f"case: ADR bilinear 3 increments, {case.nBonds} bonds", flush=True)
+    run = oracle.OracleRun(case)
+    fam = case.bodies[0].family.astype(int) - 1
+    ref = case.referenceNode - 1
+    with Solver(case, _lib.FP64) as sol:
+        for inc in range(3):
+            sol.step(ipi)
+            run.step(ipi)
+            h = sol.history()
+            R = (run.F[fam, 2] * case.cellVolume).sum()
+            assert R > 100 * (inc + 1)                       # loaded: the supports carry the beam
+            assert h["reaction_static"][-1] == pytest.approx(R, rel=1e-4)
+            assert h["deflection"][-1] == pytest.approx(run.u[ref, 2], rel=1e-4)
+            assert abs(int(sol.bonds()[1].sum()) - int(run.flag.sum())) <= 2
         obs = sol.observables()
-        Fz = (run.F[:, 2][case.bodies[0].family.astype(int) - 1] * case.cellVolume).sum()
-        assert obs["reaction_static"] == pytest.approx(Fz, rel=1e-10)
-        assert obs["cn"] > 0
+        assert obs["step"] == 3 * ipi
 
 
 def test_full_beam_dx5_bit_exact(gpu, oracle):

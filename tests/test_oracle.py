@@ -237,6 +237,24 @@ def test_adr_damping_rayleigh_quotient(oracle):
     assert oracle.calculatedampingcoefficient(F, np.array([m]), Fp, DT, vp, z3(1)) == 0.0
 
 
+def test_adr_damping_skips_driven_nodes(oracle):
+    # node 0: a free spring DOF (cn = 2 sqrt(k/m)); node 1: a node a contact body moved, with a tiny
+    # correction velocity and a large force jump.  Left in, node 1 swamps the quotient; masked out,
+    # the quotient is node 0's alone, and with nothing masked it equals the reference routine.
+    kk, m, DT = 0.25, np.array([1.0, 1.0]), 1.0
+    u_prev, u = 0.5, 0.75
+    F = np.asfortranarray([[-kk * u, 0.0, 0.0], [0.0, 0.0, 50.0]])
+    Fp = np.asfortranarray([[-kk * u_prev, 0.0, 0.0], [0.0, 0.0, -50.0]])
+    vp = np.asfortranarray([[(u - u_prev) / DT, 0.0, 0.0], [0.0, 0.0, 1e-6]])
+    uu = np.asfortranarray([[u, 0.0, 0.0], [0.0, 0.0, 2.0]])
+    full = oracle.calculatedampingcoefficient(F, m, Fp, DT, vp, uu)
+    assert full == 0.0                                  # the driven term is negative and huge
+    assert oracle.calculatedampingcoefficient_masked(F, m, Fp, DT, vp, uu, [0, 0]) == full
+    cn = oracle.calculatedampingcoefficient_masked(F, m, Fp, DT, vp, uu, [0, 1])
+    assert cn == pytest.approx(2 * math.sqrt(kk / m[0]), rel=1e-14)
+    assert oracle.calculatedampingcoefficient_masked(F, m, Fp, DT, vp, uu, [1, 1]) == 0.0   # cn2 == 0
+
+
 def test_adr_update_recurrence(oracle):
     DT, m = 1.0, np.array([4.0])
     F = np.asfortranarray([[8.0, -4.0, 0.0]])
