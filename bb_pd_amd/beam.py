@@ -151,7 +151,12 @@ def build_beam(name: str, nx: int, ny: int, nz: int, DX: float, *, law: int, hor
                seed: int = 0, sc: float | None = None, s1: float = 0.0, GF: float = 125.2, k: float = 25.0,
                alpha: float = 0.25, integrator: int = _lib.ADR, supports_x=None, penetrator_x=None,
                schedule: int = 1, increment: float = -5e-6, iters_per_increment: int = 1000,
-               nTimeSteps: int = 20000, appliedDisplacement: float = -1e-3) -> BeamCase:
+               nTimeSteps: int = 20000, appliedDisplacement: float = -1e-3,
+               pin_load_line: bool | None = None) -> BeamCase:
+    """Three-point-bend beam in the reference's arrays.  `pin_load_line` (default: on under ADR)
+    holds u_x of the top node line under the penetrator and u_y of its middle node through
+    CONSTRAINTFLAG: frictionless cylinders leave rigid x translation and rotation about z without
+    stiffness, and under ADR a drift there inflates the damping quotient's sum of u^2."""
     X, BL, L0, vcf, c, nFAM = lattice(nx, ny, nz, DX, horizon_factor, notch, E)
     nN, nB = X.shape[0], BL.shape[0]
     horizon = horizon_factor * DX
@@ -183,6 +188,16 @@ def build_beam(name: str, nx: int, ny: int, nz: int, DX: float, *, law: int, hor
     dist = np.sqrt((X[fam0, 0] - penetrator_x) ** 2 + (X[fam0, 2] - pz0) ** 2)
     pen.cz = pz0 - (dist.min() - pR)
     bodies = [pen] + supports
+    cons = np.zeros((nN, 3), order="F")
+    if pin_load_line is None:
+        pin_load_line = integrator == _lib.ADR
+    if pin_load_line:
+        line = np.isclose(X[:, 0], penetrator_x) & np.isclose(X[:, 2], nz * DX)
+        if not line.any():
+            raise ValueError("the penetrator is not above a node column: nothing to pin")
+        cons[line, 0] = 1
+        ids = np.nonzero(line)[0]
+        cons[ids[len(ids) // 2], 1] = 1
     # time step
     if integrator == _lib.EULER_CROMER:
         DT = (0.8 * math.sqrt(2 * rho * DX / (math.pi * horizon ** 2 * DX * c0))) / 1
@@ -202,7 +217,7 @@ def build_beam(name: str, nx: int, ny: int, nz: int, DX: float, *, law: int, hor
     return BeamCase(
         name=name, nx=nx, ny=ny, nz=nz, DX=DX, horizon=horizon, RADIJ=RADIJ, cellVolume=V, X=X, BONDLIST=BL,
         L0=L0, vcf=vcf, c=c, nFAM=nFAM, BONDTYPE=np.zeros(nB), fail=np.ones(nB), DENSITY=np.full(nN, rho),
-        CONSTRAINTFLAG=np.zeros((nN, 3), order="F"), c0=c0, E=E, law=law, s0=s0v,
+        CONSTRAINTFLAG=cons, c0=c0, E=E, law=law, s0=s0v,
         sc=scv if scv is not None else 0.0, s1=s1, k=k, alpha=alpha, integrator=integrator, DT=DT, mass=mass,
         bodies=bodies, schedule=schedule, nTimeSteps=nTimeSteps, appliedDisplacement=appliedDisplacement,
         increment=increment, itersPerIncrement=iters_per_increment, referenceNode=ref,

@@ -2,13 +2,16 @@
 
 Workload: the mixed-mode three-point-bend beam 500x100x50 mm at dx = 1.25 mm (400x40x80 =
 1,280,000 nodes, 74.5M bonds, offset notch, exponential softening), ADR with the penetrator
-staircase of 5e-6 m per 1000 iterations; one step = stretch -> law -> force -> nodal gather ->
-ADR update -> contact.  Both forms run on the same steps: FP32-mixed (headline `value`) and plain
-FP64.
+staircase of 5e-6 m per 1000 iterations over 20,000 iterations (elastic loading, peak and
+softening); one step = stretch -> law -> force -> nodal gather -> ADR update -> contact.  Both forms
+run the whole history: FP32-mixed (headline `value`) and plain FP64.
 
 Reported (one JSON line on stdout):
   value            bond updates/s of the FP32-mixed form = nBonds * K / seconds (inputs resident)
   fp64             the same for the plain FP64 form
+  precision        FP32-mixed against FP64 on the load history: peak of the increment-averaged load
+                   curve, the FP64 run's own rounding sensitivity beside it, and per-result errors
+                   (load, nodal force, flags, broken bonds, damage field) from identical FP64 states
   roofline         dominant kernel (bond kernel, mixed form): the byte model of the hot-path survey,
                    E*(4 + w_state + w_kappa + n_par*w) + N*180 with E = 2*nBonds entries, divided by
                    the kernel's average launch time from HIP events on the solver's stream, against
@@ -52,35 +55,152 @@ def survey_model_bytes(case, mixed: bool) -> int:
     return E * (4 + w + w + n_par * w) + case.nNodes * 180
 
 
-def run_form(case, precision, steps, warmup):
+def run_form(case, precision, steps, warmup, chunk=None, keep=()):
+    """Build the solver, run `warmup` untimed steps, then exactly `steps` timed steps in chunks of
+    `chunk` (each bracketed by synchronisations; the times are summed).  Between chunks, untimed,
+    the state at the step counts in `keep` is checkpointed to host memory.  Returns the solver (open)
+    with the timing, the whole per-step history from step 1 and the checkpoints."""
     import torch
     from bb_pd_amd import _lib
     from bb_pd_amd.solver import Solver
     t0 = time.time()
     sol = Solver(case, precision)
     log(f"  setup {time.time() - t0:.1f}s, info {sol.info()}")
+    chunk = chunk or steps
+    keep = set(keep)
+    ckpts = {}
+    rows = []
     sol.step(warmup)
     sol.sync()
-    sol.history()  # discard the warm-up record
+    if warmup in keep:
+        ckpts[warmup] = sol.checkpoint()
     sol.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sol.step(steps)
-    sol.sync()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt, done = 0.0, 0
+    while done < steps:
+        # chunks end on multiples of `chunk` counted from step 0 (load-increment boundaries)
+        at = warmup + done
+        n = min((at // chunk + 1) * chunk - at, steps - done)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol.step(n)
+        sol.sync()
+        torch.cuda.synchronize()
+        dt += time.perf_counter() - t0
+        done += n
+        rows.append(sol.history())
+        if len(rows) % 10 == 0:
+            log(f"    {warmup + done} steps")
+        if warmup + done in keep:
+            ckpts[warmup + done] = sol.checkpoint()
     avg_ms, launches = sol.kernel_time()
     sol.set_timing(False)
     info = sol.info()
-    obs = sol.observables()
-    reaction = sol.history()["reaction_static"]
-    sol.close()
+    hist = {k: np.concatenate([r[k] for r in rows]) for k in rows[0]}
     mixed = precision == _lib.MIXED
     model = survey_model_bytes(case, mixed)
     achieved = model / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    return dict(seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=case.nBonds * steps / dt,
+    return dict(solver=sol, seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=case.nBonds * steps / dt,
                 kernel_ms=avg_ms, launches=launches, bytes_model=model, bytes_layout=info["bond_kernel_bytes"],
-                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, obs=obs, info=info, reaction=reaction)
+                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, info=info, hist=hist, ckpts=ckpts)
+
+
+def increment_curve(R: np.ndarray, ipi: int) -> np.ndarray:
+    """Load per increment: the static reaction averaged over the last half of each increment's
+    iterations (row 0 = step 1)."""
+    n = len(R) // ipi
+    h = ipi // 2
+    return np.array([R[i * ipi + ipi - h:(i + 1) * ipi].mean() for i in range(n)])
+
+
+def damage_field(case, H, fail):
+    """calculatedamage.m:34-45 with broken = (bondSofteningFactor == 1) or (fail == 0)."""
+    unbroken = ~((H == 1) | (fail == 0))
+    i = case.BONDLIST[:, 0].astype(np.int64) - 1
+    j = case.BONDLIST[:, 1].astype(np.int64) - 1
+    cnt = np.bincount(i, weights=unbroken, minlength=case.nNodes) + np.bincount(j, weights=unbroken,
+                                                                              minlength=case.nNodes)
+    return 1 - cnt / case.nFAM
+
+
+def one_step_from(sol, st):
+    """Restore a checkpoint, run one step, return (nodal force, static reaction, H, flag, fail)."""
+    sol.history()
+    sol.restore(st["step"], **{k: v for k, v in st.items() if k != "step"})
+    sol.step(1)
+    sol.sync()
+    R = float(sol.history()["reaction_static"][-1])
+    F = sol.nodal()[2]
+    H, flag, fail = sol.bonds()
+    return F, R, H, flag, fail
+
+
+def precision_study(case, fp64, mixed, ipi):
+    """FP32-mixed against FP64 on the load history (the load is the penetrator family's static
+    reaction, calculatereactionforceFast.m:3-5).
+
+    * free-running: peak of the increment-averaged load curve of each form;
+    * rounding floor: the FP64 run restarted after increment 1 with one coordinate moved by one ulp
+      and run to the end; its peak difference is what rounding alone does to this (unconverged) ADR
+      load history;
+    * same state: from FP64 states along the history, incl. the FP64 peak increment, one step of each
+      form; errors of the load, nodal forces, flags, broken bonds and damage field."""
+    A, B = fp64["solver"], mixed["solver"]
+    total = len(fp64["hist"]["reaction_static"])
+    c64 = increment_curve(fp64["hist"]["reaction_static"], ipi)
+    cm = increment_curve(mixed["hist"]["reaction_static"], ipi)
+    p64, pm = int(np.argmax(c64)), int(np.argmax(cm))
+    # same-state samples: the checkpoints plus the state at the end of the FP64 peak increment
+    ck = dict(fp64["ckpts"])
+    peak_step = (p64 + 1) * ipi
+    if peak_step not in ck:
+        base = max(k for k in ck if k <= peak_step)
+        A.history()
+        A.restore(base, **{k: v for k, v in ck[base].items() if k != "step"})
+        A.step(peak_step - base)
+        A.sync()
+        ck[peak_step] = A.checkpoint()
+    same = []
+    for k in sorted(ck):
+        st = ck[k]
+        Fa, Ra, Ha, fla, faa = one_step_from(A, st)
+        Fb, Rb, Hb, flb, fab = one_step_from(B, st)
+        Da, Db = damage_field(case, Ha, faa), damage_field(case, Hb, fab)
+        same.append(dict(step=k, load=Ra, load_err=(Rb - Ra) / Ra if Ra else None,
+                         nodal_force_err=float(np.abs(Fb - Fa).max() / np.abs(Fa).max()),
+                         flag_flips=int((fla != flb).sum()),
+                         broken_fp64=int(((Ha == 1) | (faa == 0)).sum()),
+                         broken_mixed=int(((Hb == 1) | (fab == 0)).sum()),
+                         damage_err=float(np.abs(Db - Da).max())))
+        log(f"  same state at step {k}: {same[-1]}")
+    # rounding floor: restart FP64 after increment 1 with one ulp on the largest deflection
+    st = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in ck[ipi].items()}
+    i = int(np.argmax(np.abs(st["u"][:, 2])))
+    st["x"][i, 2] = np.nextafter(st["x"][i, 2], np.inf)
+    A.history()
+    A.restore(st["step"], **{k: v for k, v in st.items() if k != "step"})
+    left = total - st["step"]
+    while left > 0:
+        n = min(10 * ipi, left)
+        A.step(n)
+        A.sync()
+        left -= n
+        log(f"    rounding-floor run: {total - left} steps")
+    R2 = np.concatenate([fp64["hist"]["reaction_static"][:st["step"]], A.history()["reaction_static"]])
+    c2 = increment_curve(R2, ipi)
+    p2 = int(np.argmax(c2))
+    peak = c64[p64]
+    at_peak = next(x for x in same if x["step"] == peak_step)
+    return dict(
+        load="static reaction of the penetrator family, mean over the last half of each load increment",
+        increments=len(c64), iterations_per_increment=ipi,
+        curve_fp64=[round(float(v), 3) for v in c64], curve_mixed=[round(float(v), 3) for v in cm],
+        peak_fp64=float(peak), peak_increment_fp64=p64 + 1, peak_mixed=float(cm[pm]), peak_increment_mixed=pm + 1,
+        peak_load_err=float((cm[pm] - peak) / peak),
+        fp64_rounding_floor=float((c2[p2] - peak) / peak),
+        curve_err_max=float(np.abs(cm - c64).max() / peak),
+        curve_floor_max=float(np.abs(c2 - c64).max() / peak),
+        same_state_peak_load_err=at_peak["load_err"],
+        same_state=same)
 
 
 def cpu_baseline(case, target_s=20.0):
@@ -142,13 +262,18 @@ def measure_traffic(form: str):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=19980,
+                    help="timed steps (default: the rest of the 20,000-iteration load history)")
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic measurement")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-mixed vs FP64 load study")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
                     help="which precision forms to run (profiling runs one)")
     ap.add_argument("--small", action="store_true", help="dx = 2.5 mm notched trilinear beam instead")
+    ap.add_argument("--iters-per-increment", type=int, default=1000,
+                    help="ADR iterations per load increment (the load protocol)")
+    ap.add_argument("--increment-um", type=float, default=5.0, help="penetrator travel per increment (um)")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="only copy a device buffer of this many GiB 5 times (scales HBM counters)")
     args = ap.parse_args()
@@ -168,31 +293,47 @@ def main():
         sys.exit(2)
     from bb_pd_amd import _lib, beam
     t0 = time.time()
-    case = beam.case_trilinear_dx2p5() if args.small else beam.case_exponential_dx1p25()
+    proto = dict(iters_per_increment=args.iters_per_increment, increment=-args.increment_um * 1e-6)
+    case = beam.case_trilinear_dx2p5(**proto) if args.small else beam.case_exponential_dx1p25(**proto)
+    ipi = case.itersPerIncrement
+    total = args.warmup + args.steps
     log(f"built {case.name}: {case.nNodes} nodes {case.nBonds} bonds in {time.time() - t0:.1f}s")
     # the rocprofv3 child runs go first, while this process has not touched the GPU
     traffic = None if (args.no_traffic or args.form != "both") else measure_traffic("mixed")
     if traffic:
         log(f"traffic: {traffic['bytes'] / 1e9:.3f} GB per launch (fetch x{traffic['fetch_scale']:.2f}, "
             f"write x{traffic['write_scale']:.2f})")
+    study = args.form == "both" and not args.no_precision and total >= 2 * ipi
+    # FP64 states kept for the precision study: after increment 1 and every fifth increment
+    keep = sorted({ipi * j for j in range(1, total // ipi + 1) if j == 1 or j % 5 == 0}) if study else ()
+    chunk = ipi if study else None
     fp64 = mixed = None
     if args.form in ("both", "fp64"):
-        fp64 = run_form(case, _lib.FP64, args.steps, args.warmup)
+        fp64 = run_form(case, _lib.FP64, args.steps, args.warmup, chunk, keep)
         log(f"fp64: {fp64['ms_per_step']:.3f} ms/step, kernel {fp64['kernel_ms']:.3f} ms, "
             f"{fp64['achieved_gbs']:.0f} GB/s (model bytes)")
     if args.form in ("both", "mixed"):
-        mixed = run_form(case, _lib.MIXED, args.steps, args.warmup)
+        mixed = run_form(case, _lib.MIXED, args.steps, args.warmup, chunk)
         log(f"mixed: {mixed['ms_per_step']:.3f} ms/step, kernel {mixed['kernel_ms']:.3f} ms, "
             f"{mixed['achieved_gbs']:.0f} GB/s (model bytes)")
     if args.form != "both":
         res = fp64 or mixed
+        res["solver"].close()
         print(json.dumps({"form": args.form, "result": {k: v for k, v in res.items()
-                                                         if k not in ("obs", "info", "reaction")}}), flush=True)
+                                                         if k not in ("solver", "info", "hist", "ckpts")}}),
+              flush=True)
         return
-    # FP32-mixed against FP64 on the same steps: the load record (static reaction of the penetrator)
-    ra, rb = fp64["reaction"], mixed["reaction"]
-    peak = float(np.max(np.abs(ra))) if ra.size else 0.0
-    load_err = float(np.max(np.abs(ra - rb)) / peak) if peak > 0 else None
+    prec = None
+    if study:
+        t0 = time.time()
+        prec = precision_study(case, fp64, mixed, ipi)
+        log(f"precision study {time.time() - t0:.0f}s: peak fp64 {prec['peak_fp64']:.1f} (inc "
+            f"{prec['peak_increment_fp64']}), mixed {prec['peak_mixed']:.1f}, err {prec['peak_load_err']:.2e}, "
+            f"fp64 rounding floor {prec['fp64_rounding_floor']:.2e}, same-state peak err "
+            f"{prec['same_state_peak_load_err']:.2e}")
+    fp64["solver"].close()
+    mixed["solver"].close()
+    n_hist = int(fp64["hist"]["reaction_static"].size)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(case)
@@ -217,13 +358,16 @@ def main():
             "bonds": case.nBonds,
             "law": "exponential (k=25, alpha=0.25)" if case.law == _lib.LAW_EXPONENTIAL else "trilinear",
             "integrator": "ADR",
-            "state": f"steps {args.warmup + 1}..{args.warmup + args.steps} of the load history (elastic)",
+            "load": f"penetrator staircase {abs(case.increment) * 1e6:g} um per {ipi} iterations",
+            "state": f"steps {args.warmup + 1}..{total} of a {n_hist}-step load history (elastic, peak, "
+                     f"softening)",
         },
         "fp64": {"value": fp64["bond_updates_per_s"], "ms_per_step": fp64["ms_per_step"],
                  "kernel_ms": fp64["kernel_ms"], "bytes_model": fp64["bytes_model"],
                  "bytes_layout": fp64["bytes_layout"], "achieved_gbs": fp64["achieved_gbs"], "frac": fp64["frac"]},
         "mixed_over_fp64": head["bond_updates_per_s"] / fp64["bond_updates_per_s"],
-        "mixed_vs_fp64_load_err": load_err,
+        "mixed_vs_fp64_load_err": prec["same_state_peak_load_err"] if prec else None,
+        "precision": prec,
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["frac"], "traffic": traffic["bytes"] if traffic else None,
                      "kernel": "k_bond (FP32-mixed)", "kernel_ms": head["kernel_ms"],
