@@ -88,6 +88,7 @@ SIGNATURES = {
     "bbpd_solver_restore": (_I, [_V, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "bbpd_solver_history": (_I, [_V, _P, _I, C.POINTER(_I)]),
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
+    "bbpd_solver_layout": (C.c_char_p, [_V]),
     "bbpd_solver_set_timing": (_I, [_V, _I]),
     "bbpd_solver_kernel_time": (_I, [_V, _dp, C.POINTER(_L)]),
 }

@@ -53,8 +53,11 @@ constexpr uint32_t FLAG_BIT = 1u << 31;
 constexpr int HIST_CAP = 1 << 20;
 constexpr int N_HIST = 4;      // reaction, static reaction, deflection, CMOD
 constexpr int TAB_MAX = 2048;  // entries of one LDS parameter table
-constexpr int LAT_BITS = 21;   // bits per packed lattice coordinate
+constexpr int LAT_BITS = 18;   // bits per packed lattice coordinate
 constexpr uint64_t LAT_MASK = (1ull << LAT_BITS) - 1;
+constexpr int G_SHIFT = 56;    // node record: the node's bond count (nFAMILYMEMBERS) in the top byte
+constexpr int ST_MAX = 128;    // offsets of the stencil layout (bits of the per-node mask)
+constexpr int ST_GMAX = 45;    // distinct family sizes of the stencil layout (key table <= 45^2)
 // node-flag bits 0-2: DOF d constrained; bit 3: member of a contact family (its DOFs are driven by
 // the rigid-body projection, so the ADR Rayleigh quotient leaves the node out)
 constexpr unsigned char CONTACT_BIT = 1u << 3;
@@ -75,7 +78,7 @@ struct DevScalars {
 
 struct __align__(32) NodeRec {
     double x, y, z; // position: deformed coordinates (FP64 form) or displacement (mixed form)
-    uint64_t q;     // packed lattice coordinates kx | ky << 21 | kz << 42
+    uint64_t q;     // packed lattice coordinates kx | ky << 18 | kz << 36, bond count << 56
 };
 
 struct Params {
@@ -101,12 +104,23 @@ struct Params {
     int ntabc, ntabp;
     int cmode, s0mode, scmode;
     int adr;
+    int law_linear; // the brittle latch: the stencil layout's sign bit is the failed flag
     // scalars
     double V, horizon, hmr, hpr, two_r; // hmr = horizon - RADIJ, hpr = horizon + RADIJ, two_r = 2*RADIJ
     double s0, s1, sc, k, alpha, denom_exp, sc_steel, ff;
     double DT, D;
     double *partials;
     DevScalars *ds;
+    // stencil layout (full lattice box, rows in lattice-offset order): neighbours are implicit
+    int noff;             // offsets, padded to even
+    const int4 *soff;     // per offset: slot delta, dx, dy, dz
+    const float4 *sxi;    // per offset (mixed form): xi = h * d in FP32, |xi|
+    const float *svcf;    // per offset (mixed form): volume correction factor of |h * d|
+    const uint4 *smask;   // per slot: bit o set when the node has the bond at offset o
+    void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
+    const float *tabf;    // mixed-form key tables (c * V, then parameter tables)
+    int ts0, tsc;         // start of the s0 / sc key table inside the tables
+    int kg;               // key = gidx_n * kg + gidx_m
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -177,6 +191,83 @@ __device__ inline int remap_block(int b, int nb)
 __device__ inline uint32_t *rec_word(uint4 *rec, long long e, int word)
 {
     return reinterpret_cast<uint32_t *>(rec + e) + word;
+}
+
+// After a node's row: the Euler-Cromer update of the node (timeintegrationeulercromerMex.c:38-46),
+// writing its next position record, or, for ADR, F and the block's Rayleigh-quotient partial sums
+// (calculatedampingcoefficient.m:61-85).  Every thread of the block calls it (block reduction).
+template <bool MIXED>
+__device__ __forceinline__ void finish_node(const Params &P, int n, int b, int cur, double f0, double f1, double f2,
+                                            double qn0, double qn1, double qn2, double Xn0, double Xn1, double Xn2,
+                                            double *sh)
+{
+    const bool real = n < P.nN;
+    if (!P.adr) {
+        if (!real) return;
+        const int nxt = cur ^ 1;
+        P.F0[n] = f0;
+        P.F1[n] = f1;
+        P.F2[n] = f2;
+        const unsigned char cf = P.cons[n];
+        const double r = P.rho[n];
+        double *uu[3] = {P.u0, P.u1, P.u2};
+        double *vv[3] = {P.v0, P.v1, P.v2};
+        const double Xn[3] = {Xn0, Xn1, Xn2};
+        const double qn[3] = {qn0, qn1, qn2};
+        const double ff[3] = {f0, f1, f2};
+        double out[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double a = (ff[d] - P.D * vv[d][n]) / r;
+            if (cf & (1u << d)) a = 0;
+            const double nv = vv[d][n] + (a * P.DT);
+            vv[d][n] = nv;
+            if constexpr (MIXED) {
+                out[d] = qn[d] + (nv * P.DT);
+            } else {
+                const double nu = uu[d][n] + (nv * P.DT);
+                uu[d][n] = nu;
+                out[d] = Xn[d] + nu;
+            }
+        }
+        NodeRec *po = P.pos[nxt] + n;
+        po->x = out[0];
+        po->y = out[1];
+        po->z = out[2];
+    } else {
+        double c1 = 0, c2 = 0;
+        if (real) {
+            P.F0[n] = f0;
+            P.F1[n] = f1;
+            P.F2[n] = f2;
+        }
+        // the quotient runs over the nodes no contact body drives: a family member's velocity
+        // after a projection is the projection's correction, and its term would outweigh the
+        // structure's by orders of magnitude, pinning cn to 0 or the clamp
+        if (real && !(P.cons[n] & CONTACT_BIT)) {
+            const double mi = P.mass[n];
+            const double ff[3] = {f0, f1, f2};
+            const double fp[3] = {P.Fp0[n], P.Fp1[n], P.Fp2[n]};
+            const double vp[3] = {P.v0[n], P.v1[n], P.v2[n]};
+            double uq[3];
+            if constexpr (MIXED) {
+                uq[0] = qn0, uq[1] = qn1, uq[2] = qn2;
+            } else {
+                uq[0] = P.u0[n], uq[1] = P.u1[n], uq[2] = P.u2[n];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                if (vp[d] != 0) c1 = c1 - (uq[d] * uq[d]) * ((ff[d] / mi - fp[d] / mi) / (P.DT * vp[d]));
+                c2 = c2 + uq[d] * uq[d];
+            }
+        }
+        const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
+        const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
+        if (threadIdx.x == 0) {
+            P.partials[2 * b] = s1;
+            P.partials[2 * b + 1] = s2;
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -260,7 +351,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             const uint64_t q = d.nb.q;
             Xm0 = P.hx * (double)(int)(q & LAT_MASK);
             Xm1 = P.hy * (double)(int)((q >> LAT_BITS) & LAT_MASK);
-            Xm2 = P.hz * (double)(int)(q >> (2 * LAT_BITS));
+            Xm2 = P.hz * (double)(int)((q >> (2 * LAT_BITS)) & LAT_MASK);
         } else {
             Xm0 = d.b0, Xm1 = d.b1, Xm2 = d.b2;
         }
@@ -349,73 +440,169 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             iB = iB2;
         }
     }
-    const bool real = n < P.nN;
-    if (!P.adr) {
-        if (!real) return;
-        const int nxt = cur ^ 1;
-        P.F0[n] = f0;
-        P.F1[n] = f1;
-        P.F2[n] = f2;
-        const unsigned char cf = P.cons[n];
-        const double r = P.rho[n];
-        double *uu[3] = {P.u0, P.u1, P.u2};
-        double *vv[3] = {P.v0, P.v1, P.v2};
-        const double Xn[3] = {Xn0, Xn1, Xn2};
-        const double qn[3] = {qn0, qn1, qn2};
-        const double ff[3] = {f0, f1, f2};
-        double out[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double a = (ff[d] - P.D * vv[d][n]) / r;
-            if (cf & (1u << d)) a = 0;
-            const double nv = vv[d][n] + (a * P.DT);
-            vv[d][n] = nv;
-            if constexpr (MIXED) {
-                out[d] = qn[d] + (nv * P.DT);
-            } else {
-                const double nu = uu[d][n] + (nv * P.DT);
-                uu[d][n] = nu;
-                out[d] = Xn[d] + nu;
-            }
-        }
-        NodeRec *po = P.pos[nxt] + n;
-        po->x = out[0];
-        po->y = out[1];
-        po->z = out[2];
+    finish_node<MIXED>(P, n, b, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Bond kernel, stencil layout.  On a full lattice box whose rows list their bonds in ascending
+// lattice-offset order (the order a BONDLIST sorted by node id gives), the neighbour at offset o of
+// slot n is slot n + delta_o: nothing per entry but the history is stored.  Per node, a 128-bit
+// mask says which offsets are bonds (boundaries, notch).  BONDSTIFFNESS is a function of the pair
+// (nFAMILY_i, nFAMILY_j) (buildbonddata.m:48-51), read from a small table keyed by the pair of the
+// two nodes' family-size indices; per-bond s0 / sc come from key tables as well when they are
+// functions of that key, else from per-entry streams.
+// The softening flag (failed flag for the brittle law) is the sign bit of the stored history.
+// Entries are visited in offset order, so the sums per node are those of the record layout and of
+// the reference's serial nodal loop, bit for bit (FP64).
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+struct StEnt {
+    T h;        // stored history (sign = flag)
+    T p0, p1;   // streamed s0, sc
+    NodeRec nb; // neighbour record
+};
+
+template <bool MIXED, int LAW>
+__global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
+{
+    using T = typename std::conditional<MIXED, float, double>::type;
+    __shared__ double sh[BLOCK];
+    // per-offset data in LDS: a neighbour address depends on its offset's slot delta, and an LDS
+    // read is far shorter than the global load it would otherwise wait for
+    __shared__ int4 s_off[ST_MAX];
+    __shared__ float4 s_xi[MIXED ? ST_MAX : 1];
+    __shared__ float s_vcf[MIXED ? ST_MAX : 1];
+    extern __shared__ double tabs[]; // key tables (FP64, or FP32 in the mixed form)
+    const int b = remap_block(blockIdx.x, gridDim.x);
+    const int n = b * BLOCK + threadIdx.x;
+    const int cur = (int)(P.ds->step & 1);
+    const NodeRec *__restrict__ pos = P.pos[cur];
+    const int ntab = P.ntabc + P.ntabp;
+    if constexpr (MIXED) {
+        float *tf = reinterpret_cast<float *>(tabs);
+        for (int i = threadIdx.x; i < ntab; i += BLOCK) tf[i] = P.tabf[i];
     } else {
-        double c1 = 0, c2 = 0;
-        if (real) {
-            P.F0[n] = f0;
-            P.F1[n] = f1;
-            P.F2[n] = f2;
-        }
-        // the quotient runs over the nodes no contact body drives: a family member's velocity
-        // after a projection is the projection's correction, and its term would outweigh the
-        // structure's by orders of magnitude, pinning cn to 0 or the clamp
-        if (real && !(P.cons[n] & CONTACT_BIT)) {
-            const double mi = P.mass[n];
-            const double ff[3] = {f0, f1, f2};
-            const double fp[3] = {P.Fp0[n], P.Fp1[n], P.Fp2[n]};
-            const double vp[3] = {P.v0[n], P.v1[n], P.v2[n]};
-            double uq[3];
-            if constexpr (MIXED) {
-                uq[0] = qn0, uq[1] = qn1, uq[2] = qn2;
-            } else {
-                uq[0] = P.u0[n], uq[1] = P.u1[n], uq[2] = P.u2[n];
-            }
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                if (vp[d] != 0) c1 = c1 - (uq[d] * uq[d]) * ((ff[d] / mi - fp[d] / mi) / (P.DT * vp[d]));
-                c2 = c2 + uq[d] * uq[d];
-            }
-        }
-        const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
-        const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
-        if (threadIdx.x == 0) {
-            P.partials[2 * b] = s1;
-            P.partials[2 * b + 1] = s2;
+        for (int i = threadIdx.x; i < ntab; i += BLOCK) tabs[i] = P.tab[i];
+    }
+    for (int i = threadIdx.x; i < P.noff; i += BLOCK) {
+        s_off[i] = P.soff[i];
+        if constexpr (MIXED) {
+            s_xi[i] = P.sxi[i];
+            s_vcf[i] = P.svcf[i];
         }
     }
+    __syncthreads();
+    const T *tab = reinterpret_cast<const T *>(tabs);
+
+    const NodeRec me = pos[n];
+    const double qn0 = me.x, qn1 = me.y, qn2 = me.z;
+    const int kx = (int)(me.q & LAT_MASK), ky = (int)((me.q >> LAT_BITS) & LAT_MASK),
+              kz = (int)((me.q >> (2 * LAT_BITS)) & LAT_MASK);
+    const int gme = (int)(me.q >> G_SHIFT); // index of the node's family size
+    // X == h * k exactly (checked at set-up)
+    const double Xn0 = P.hx * (double)kx, Xn1 = P.hy * (double)ky, Xn2 = P.hz * (double)kz;
+    const uint4 msk = P.smask[n];
+    // the bond mask, shifted right as the loop advances: bit 0 is the offset being computed
+    unsigned __int128 mk = ((unsigned __int128)(((uint64_t)msk.w << 32) | msk.z) << 64) |
+                           (((uint64_t)msk.y << 32) | msk.x);
+    T *__restrict__ hs = reinterpret_cast<T *>(P.hs);
+    const int noff = P.noff;
+    const long long ebase = (long long)(n >> 6) * noff * SLICE + (n & (SLICE - 1));
+    double f0 = 0, f1 = 0, f2 = 0;
+    // `bit` = this lane has the bond at offset o
+    auto load = [&](StEnt<T> &d, int o, bool bit) {
+        o = min(o, noff - 1);
+        const long long e = ebase + (long long)o * SLICE;
+        const int m = bit ? n + s_off[o].x : n; // absent bonds read the node itself (valid memory)
+        d.h = __builtin_nontemporal_load(hs + e);
+        if (P.s0mode == PM_STREAM) d.p0 = __builtin_nontemporal_load((const T *)P.s0str + e);
+        if (P.scmode == PM_STREAM) d.p1 = __builtin_nontemporal_load((const T *)P.scstr + e);
+        d.nb = pos[m];
+    };
+    auto compute = [&](const StEnt<T> &d, int o, bool bit) {
+        if (!bit) return;
+        const long long e = ebase + (long long)o * SLICE;
+        const int key = gme * P.kg + (int)(d.nb.q >> G_SHIFT);
+        const bool sgn = MIXED ? (__float_as_uint((float)d.h) >> 31) != 0u
+                               : (__double_as_longlong((double)d.h) < 0);
+        const uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
+        uint32_t w = sgn ? mark : 0u;
+        T H = d.h < T(0) ? -d.h : d.h; // |h|, and -0 -> +0
+        if (H == T(0)) H = T(0);
+        const T s0 = P.s0mode == PM_TABLE ? tab[P.ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : (T)P.s0);
+        const T sc = P.scmode == PM_TABLE ? tab[P.tsc + key] : (P.scmode == PM_STREAM ? d.p1 : (T)P.sc);
+        if constexpr (!MIXED) {
+            const int4 so = s_off[o];
+            const double Xm0 = P.hx * (double)(kx + so.y), Xm1 = P.hy * (double)(ky + so.z),
+                         Xm2 = P.hz * (double)(kz + so.w);
+            const double c = tab[key];
+            // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
+            const double dx = d.nb.x - qn0, dy = d.nb.y - qn1, dz = d.nb.z - qn2;
+            const double l = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
+            const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+            const double s = (l - L0) / L0;
+            apply_law<LAW, double>(s, s0, sc, P.s1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
+            double vcf; // calculatevolumecorrectionfactors.m:20-32
+            if (L0 <= P.hmr) vcf = 1;
+            else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
+            else vcf = 0;
+            const double fail = (w & FAILED_BIT) ? 0.0 : 1.0;
+            const double p = fail * c * (1 - H) * s * P.V * vcf;
+            // correctly rounded quotients dx/l etc. (see k_bond)
+            const double rl = 1.0 / l;
+            double qx = dx * rl, qy = dy * rl, qz = dz * rl;
+            qx = fma(fma(-l, qx, dx), rl, qx);
+            qy = fma(fma(-l, qy, dy), rl, qy);
+            qz = fma(fma(-l, qz, dz), rl, qz);
+            double gx = p * qx, gy = p * qy, gz = p * qz;
+            if (gx != gx) gx = 0.0;
+            if (gy != gy) gy = 0.0;
+            if (gz != gz) gz = 0.0;
+            f0 += gx;
+            f1 += gy;
+            f2 += gz;
+            const double hn = (w & mark) ? -H : H;
+            if (__double_as_longlong(hn) != __double_as_longlong((double)d.h)) hs[e] = hn;
+        } else {
+            const float4 xi = s_xi[o];
+            const float kap = tab[key] * s_vcf[o];
+            const float et0 = (float)(d.nb.x - qn0), et1 = (float)(d.nb.y - qn1), et2 = (float)(d.nb.z - qn2);
+            const float y0 = xi.x + et0, y1 = xi.y + et1, y2 = xi.z + et2;
+            const float L0 = xi.w;
+            const float l = __builtin_amdgcn_sqrtf(fmaf(y0, y0, fmaf(y1, y1, y2 * y2)));
+            const float num =
+                fmaf(2.0f, fmaf(xi.x, et0, fmaf(xi.y, et1, xi.z * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
+            const float rl = __builtin_amdgcn_rcpf(l);
+            const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
+            apply_law<LAW, float>(s, s0, sc, (float)P.s1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
+                                  (float)P.sc_steel, (float)P.ff, H, w);
+            const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
+            const float p = fail * kap * (1.0f - H) * s * rl;
+            float gx = p * y0, gy = p * y1, gz = p * y2;
+            if (gx != gx) gx = 0.0f;
+            if (gy != gy) gy = 0.0f;
+            if (gz != gz) gz = 0.0f;
+            f0 += (double)gx;
+            f1 += (double)gy;
+            f2 += (double)gz;
+            const float hn = (w & mark) ? -H : H;
+            if (__float_as_uint(hn) != __float_as_uint((float)d.h)) hs[e] = hn;
+        }
+    };
+    {
+        StEnt<T> A, B;
+        load(A, 0, (uint32_t)mk & 1u);
+        for (int o = 0; o < noff; o += 2) {
+            const uint32_t b3 = (uint32_t)mk & 7u; // offsets o, o + 1, o + 2
+            load(B, o + 1, (b3 & 2u) != 0u);
+            compute(A, o, (b3 & 1u) != 0u);
+            load(A, o + 2, (b3 & 4u) != 0u);
+            compute(B, o + 1, (b3 & 2u) != 0u);
+            mk >>= 2;
+        }
+    }
+    finish_node<MIXED>(P, n, b, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
 }
 
 // ADR: damping coefficient from the block partials (calculatedampingcoefficient.m:90-112)
@@ -631,7 +818,24 @@ __global__ __launch_bounds__(BLOCK) void k_count(Params P, unsigned long long *o
 {
     const int n = blockIdx.x * BLOCK + threadIdx.x;
     unsigned long long cs = 0, cb = 0, cf = 0;
-    if (n < P.nSlots) {
+    if (n < P.nSlots && P.noff > 0) {
+        // stencil layout: history per (slot, offset), sign bit = flag (failed for the brittle law)
+        const uint4 msk = P.smask[n];
+        const uint32_t mw[4] = {msk.x, msk.y, msk.z, msk.w};
+        const long long ebase = (long long)(n >> 6) * P.noff * SLICE + (n & (SLICE - 1));
+        for (int o = 0; o < P.noff; o++) {
+            if (!((mw[o >> 5] >> (o & 31)) & 1u)) continue;
+            const long long e = ebase + (long long)o * SLICE;
+            double h;
+            if (MIXED) h = (double)reinterpret_cast<const float *>(P.hs)[e];
+            else h = reinterpret_cast<const double *>(P.hs)[e];
+            const bool sgn = std::signbit(h);
+            const bool failed = P.law_linear && sgn;
+            cs += (!P.law_linear && sgn) ? 1 : 0;
+            cf += failed ? 1 : 0;
+            cb += (failed || std::fabs(h) == 1.0) ? 1 : 0;
+        }
+    } else if (n < P.nSlots) {
         const int len = P.rowlen[n];
         long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
         for (int kk = 0; kk < len; kk++, e += SLICE) {
@@ -704,6 +908,8 @@ struct bbpd_solver {
     hipStream_t stream = nullptr;
     int precision = 0, law = 0, integrator = 0;
     bool lattice = false;
+    bool stencil = false;   // stencil layout (implicit neighbours) rather than entry records
+    std::string layout;     // which layout, and why not the stencil one
     int nN = 0, nB = 0, nSlots = 0, nSlices = 0;
     long long nEntries = 0, nPadded = 0;
     std::vector<int> slot_of;                // original node (0-based) -> slot
@@ -749,10 +955,22 @@ KernelFn pick_law(int law, bool lat)
     }
 }
 
+template <bool MIXED>
+KernelFn pick_law_st(int law)
+{
+    switch (law) {
+    case BBPD_LAW_LINEAR: return k_bond_st<MIXED, BBPD_LAW_LINEAR>;
+    case BBPD_LAW_BILINEAR: return k_bond_st<MIXED, BBPD_LAW_BILINEAR>;
+    case BBPD_LAW_TRILINEAR: return k_bond_st<MIXED, BBPD_LAW_TRILINEAR>;
+    default: return k_bond_st<MIXED, BBPD_LAW_EXPONENTIAL>;
+    }
+}
+
 KernelFn pick_bond_kernel(const bbpd_solver *s)
 {
-    if (s->precision == BBPD_PRECISION_MIXED) return pick_law<true>(s->law, s->lattice);
-    return pick_law<false>(s->law, s->lattice);
+    const bool mixed = s->precision == BBPD_PRECISION_MIXED;
+    if (s->stencil) return mixed ? pick_law_st<true>(s->law) : pick_law_st<false>(s->law);
+    return mixed ? pick_law<true>(s->law, s->lattice) : pick_law<false>(s->law, s->lattice);
 }
 
 int launch_step(bbpd_solver *s, bool allow_timing)
@@ -822,6 +1040,179 @@ bool make_table(const double *a, int n, int cap, std::vector<double> &tab, std::
         idx[k] = it->second;
     }
     return true;
+}
+
+} // namespace
+
+namespace {
+
+// Stencil layout plan.  Applies to a lattice input (X == h * k) that fills a box, whose bonds all
+// have BONDTYPE 0 (fail 1 unless the law is the brittle latch), whose rows list their bonds in
+// ascending lattice-offset order (z, then y, then x: what a BONDLIST sorted by node id on an
+// x-fastest lattice gives), with at most ST_MAX distinct offsets, and whose BONDSTIFFNESS is a
+// function of the pair (nFAMILY_i, nFAMILY_j).  Slots are then the x-fastest lattice order.
+struct StencilPlan {
+    bool ok = false;
+    std::string why;              // reason the layout does not apply
+    int noff = 0;                 // offsets, padded to even
+    std::vector<int4> off;        // slot delta, dx, dy, dz
+    std::vector<int> slot;        // node -> slot
+    std::vector<int> oi, oj;      // per bond: offset index at the i end and at the j end
+    std::vector<int> g;           // per node: bond count
+    std::vector<int> gidx;        // per node: index of its bond count among the distinct counts
+    int G = 0;                    // distinct bond counts (key = gidx_n * G + gidx_m)
+    std::vector<int> k;           // per node: lattice coordinates (absolute), 3 per node
+    int nkey = 0;
+    std::vector<double> tc, ts0, tsc; // key tables (empty when not tabled)
+    int s0mode = PM_SCALAR, scmode = PM_SCALAR;
+};
+
+bool key_table(const double *a, long long nB, const std::vector<int> &key, int nkey, std::vector<double> &tab)
+{
+    tab.assign(nkey, 0.0);
+    std::vector<char> set(nkey, 0);
+    for (long long b = 0; b < nB; b++) {
+        const int q = key[b];
+        if (!set[q]) {
+            set[q] = 1;
+            tab[q] = a[b];
+        } else if (!bits_equal(tab[q], a[b])) {
+            return false;
+        }
+    }
+    return true;
+}
+
+void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int> &bi, const std::vector<int> &bj,
+                  const double *hlat, StencilPlan &sp)
+{
+    const int nN = pr->nNodes, nB = pr->nBonds;
+    sp.ok = false;
+    sp.k.assign(3 * (size_t)nN, 0);
+    int kmin[3] = {1 << 30, 1 << 30, 1 << 30}, kmax[3] = {-1, -1, -1};
+    for (int n = 0; n < nN; n++)
+        for (int d = 0; d < 3; d++) {
+            const int kk = (int)std::floor(X[n + (size_t)nN * d] / hlat[d] + 0.5);
+            sp.k[3 * (size_t)n + d] = kk;
+            kmin[d] = std::min(kmin[d], kk);
+            kmax[d] = std::max(kmax[d], kk);
+        }
+    long long dims[3];
+    for (int d = 0; d < 3; d++) dims[d] = kmax[d] - kmin[d] + 1;
+    if (dims[0] * dims[1] * dims[2] != (long long)nN) {
+        sp.why = "the nodes do not fill a lattice box";
+        return;
+    }
+    sp.slot.assign(nN, 0);
+    std::vector<char> seen(nN, 0);
+    for (int n = 0; n < nN; n++) {
+        const long long q = (sp.k[3 * (size_t)n] - kmin[0]) +
+                            dims[0] * ((sp.k[3 * (size_t)n + 1] - kmin[1]) + dims[1] * (sp.k[3 * (size_t)n + 2] - kmin[2]));
+        if (seen[q]) {
+            sp.why = "two nodes share a lattice site";
+            return;
+        }
+        seen[q] = 1;
+        sp.slot[n] = (int)q;
+    }
+    for (int b = 0; b < nB; b++) {
+        if (pr->BONDTYPE && pr->BONDTYPE[b] != 0) {
+            sp.why = "BONDTYPE other than 0";
+            return;
+        }
+        if (pr->law != BBPD_LAW_LINEAR && pr->fail && pr->fail[b] != 1) {
+            sp.why = "fail other than 1 with a softening law";
+            return;
+        }
+    }
+    // offsets: the union over all bonds of k_j - k_i and its negation
+    int R = 0;
+    for (int b = 0; b < nB; b++)
+        for (int d = 0; d < 3; d++) R = std::max(R, std::abs(sp.k[3 * (size_t)bj[b] + d] - sp.k[3 * (size_t)bi[b] + d]));
+    if (R > 15) {
+        sp.why = "bond offsets longer than 15 lattice spacings";
+        return;
+    }
+    const int Wd = 2 * R + 1;
+    auto cell = [&](int dx, int dy, int dz) { return (dx + R) + Wd * ((dy + R) + Wd * (dz + R)); };
+    std::vector<int> idx((size_t)Wd * Wd * Wd, -1);
+    for (int b = 0; b < nB; b++) {
+        int dd[3];
+        for (int d = 0; d < 3; d++) dd[d] = sp.k[3 * (size_t)bj[b] + d] - sp.k[3 * (size_t)bi[b] + d];
+        idx[cell(dd[0], dd[1], dd[2])] = 0;
+        idx[cell(-dd[0], -dd[1], -dd[2])] = 0;
+    }
+    // ascending z, then y, then x: ascending slot delta
+    int no = 0;
+    for (int dz = -R; dz <= R; dz++)
+        for (int dy = -R; dy <= R; dy++)
+            for (int dx = -R; dx <= R; dx++)
+                if (idx[cell(dx, dy, dz)] == 0) {
+                    idx[cell(dx, dy, dz)] = no++;
+                    sp.off.push_back(make_int4((int)(dx + dims[0] * (dy + dims[1] * dz)), dx, dy, dz));
+                }
+    if (no > ST_MAX) {
+        sp.why = "more than 128 distinct bond offsets";
+        return;
+    }
+    if (no & 1) sp.off.push_back(make_int4(0, 0, 0, 0)); // padding offset, never a bond
+    sp.noff = (int)sp.off.size();
+    // rows in ascending offset order, each offset at most once per node
+    sp.oi.assign(nB, 0);
+    sp.oj.assign(nB, 0);
+    std::vector<int> last(nN, -1);
+    sp.g.assign(nN, 0);
+    for (int b = 0; b < nB; b++) {
+        const int i = bi[b], j = bj[b];
+        int dd[3];
+        for (int d = 0; d < 3; d++) dd[d] = sp.k[3 * (size_t)j + d] - sp.k[3 * (size_t)i + d];
+        const int a = idx[cell(dd[0], dd[1], dd[2])], c = idx[cell(-dd[0], -dd[1], -dd[2])];
+        if (a <= last[i] || c <= last[j]) {
+            sp.why = "rows not in ascending lattice-offset order (BONDLIST not sorted by node id)";
+            return;
+        }
+        last[i] = a;
+        last[j] = c;
+        sp.oi[b] = a;
+        sp.oj[b] = c;
+        sp.g[i]++;
+        sp.g[j]++;
+    }
+    // BONDSTIFFNESS (and s0 / sc where possible) as tables keyed by the pair of family sizes
+    // (nFAMILY_i, nFAMILY_j): each node carries the index of its size among the distinct sizes
+    std::vector<int> sizes(sp.g.begin(), sp.g.end());
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    if ((int)sizes.size() > ST_GMAX) {
+        sp.why = "more than " + std::to_string(ST_GMAX) + " distinct family sizes";
+        return;
+    }
+    sp.G = (int)sizes.size();
+    sp.gidx.assign(nN, 0);
+    for (int n = 0; n < nN; n++) sp.gidx[n] = (int)(std::lower_bound(sizes.begin(), sizes.end(), sp.g[n]) - sizes.begin());
+    sp.nkey = sp.G * sp.G;
+    // both orders of the pair index the same value (the reference's vi + vj is commutative)
+    std::vector<int> key(2 * (size_t)nB);
+    for (int b = 0; b < nB; b++) {
+        key[2 * (size_t)b] = sp.gidx[bi[b]] * sp.G + sp.gidx[bj[b]];
+        key[2 * (size_t)b + 1] = sp.gidx[bj[b]] * sp.G + sp.gidx[bi[b]];
+    }
+    auto pair_table = [&](const double *a, std::vector<double> &tab) {
+        std::vector<double> both(2 * (size_t)nB);
+        for (int b = 0; b < nB; b++) both[2 * (size_t)b] = both[2 * (size_t)b + 1] = a[b];
+        return key_table(both.data(), 2 * nB, key, sp.nkey, tab);
+    };
+    if (!pair_table(pr->BONDSTIFFNESS, sp.tc)) {
+        sp.why = "BONDSTIFFNESS is not a function of (nFAMILY_i, nFAMILY_j)";
+        return;
+    }
+    sp.s0mode = PM_SCALAR;
+    sp.scmode = PM_SCALAR;
+    if (pr->s0) sp.s0mode = pair_table(pr->s0, sp.ts0) ? PM_TABLE : PM_STREAM;
+    if (pr->sc) sp.scmode = pair_table(pr->sc, sp.tsc) ? PM_TABLE : PM_STREAM;
+    if (sp.s0mode != PM_TABLE) sp.ts0.clear();
+    if (sp.scmode != PM_TABLE) sp.tsc.clear();
+    sp.ok = true;
 }
 
 } // namespace
@@ -934,13 +1325,28 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     }
     s->lattice = lattice;
 
-    // ---- slots.  Tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in Morton order,
+    // ---- layout.  The FP32-mixed form takes the stencil layout where it applies: its bond kernel
+    // is bound by memory traffic, which the stencil layout cuts.  The plain FP64 form is bound by
+    // its FP64 arithmetic and runs a little faster on the record layout's node order, which it
+    // takes by default.  BBPD_LAYOUT=stencil / records overrides either.
+    StencilPlan sp;
+    {
+        const char *lay = std::getenv("BBPD_LAYOUT");
+        const bool want_records = lay ? std::strcmp(lay, "records") == 0 : !mixed;
+        if (!lattice) sp.why = "not a lattice";
+        else if (want_records) sp.why = lay ? "record layout requested" : "FP64 default";
+        else plan_stencil(pr, X, bi, bj, hlat, sp);
+    }
+    s->stencil = sp.ok;
+    s->layout = sp.ok ? "stencil" : "records (" + sp.why + ")";
+
+    // ---- slots.  Stencil layout: the x-fastest lattice order.  Records: tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in Morton order,
     // lines of 64 inside a tile, so a wavefront is 64 consecutive nodes of a lattice line and the
     // k-th neighbours of its lanes are again 64 consecutive slots.
     int ax[3] = {0, 1, 2}; // ax[0] = longest axis
     std::sort(ax, ax + 3, [&](int a, int b) { return (xmax[a] - xmin[a]) > (xmax[b] - xmin[b]); });
-    std::vector<uint64_t> code(nN);
-    for (int n = 0; n < nN; n++) {
+    std::vector<uint64_t> code(sp.ok ? 0 : nN);
+    for (int n = 0; n < (sp.ok ? 0 : nN); n++) {
         uint32_t q[3];
         for (int d = 0; d < 3; d++) {
             double t = std::floor((X[n + (size_t)nN * ax[d]] - xmin[ax[d]]) / hq + 0.5);
@@ -950,14 +1356,18 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         uint64_t in = ((uint64_t)((q[2] & 1u) * 2u + (q[1] & 1u)) << 6) | (q[0] & 63u);
         code[n] = (tile << 8) | in;
     }
-    std::vector<int> order(nN);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return code[a] < code[b]; });
     const int nSlots = (int)(((long long)nN + SLOT_ALIGN - 1) / SLOT_ALIGN * SLOT_ALIGN);
     s->nSlots = nSlots;
     s->nSlices = nSlots / SLICE;
-    s->slot_of.assign(nN, 0);
-    for (int q = 0; q < nN; q++) s->slot_of[order[q]] = q;
+    if (sp.ok) {
+        s->slot_of = sp.slot;
+    } else {
+        std::vector<int> order(nN);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return code[a] < code[b]; });
+        s->slot_of.assign(nN, 0);
+        for (int q = 0; q < nN; q++) s->slot_of[order[q]] = q;
+    }
 
     // ---- rows in BONDLIST order
     std::vector<int> rowlen(nSlots, 0);
@@ -965,25 +1375,42 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         rowlen[s->slot_of[bi[k]]]++;
         rowlen[s->slot_of[bj[k]]]++;
     }
+    // record layout: SELL-64 slices of the rows; stencil layout: one entry per (slot, offset)
     std::vector<long long> slice_off(s->nSlices + 1, 0);
-    for (int sl = 0; sl < s->nSlices; sl++) {
-        int w = 0;
-        for (int t = 0; t < SLICE; t++) w = std::max(w, rowlen[sl * SLICE + t]);
-        slice_off[sl + 1] = slice_off[sl] + (long long)w * SLICE;
+    long long nE = 0;
+    if (!sp.ok) {
+        for (int sl = 0; sl < s->nSlices; sl++) {
+            int w = 0;
+            for (int t = 0; t < SLICE; t++) w = std::max(w, rowlen[sl * SLICE + t]);
+            slice_off[sl + 1] = slice_off[sl] + (long long)w * SLICE;
+        }
+        nE = slice_off[s->nSlices];
+    } else {
+        nE = (long long)s->nSlices * sp.noff * SLICE;
     }
-    const long long nE = slice_off[s->nSlices];
     s->nEntries = 2LL * nB;
     s->nPadded = nE;
     std::vector<int> fillpos(nSlots, 0);
     s->entry_i.assign(nB, 0);
     s->entry_j.assign(nB, 0);
     auto entry_at = [&](int slot, int kk) { return slice_off[slot / SLICE] + (long long)kk * SLICE + (slot % SLICE); };
+    auto dense_at = [&](int slot, int o) {
+        return ((long long)(slot / SLICE) * sp.noff + o) * SLICE + (slot % SLICE);
+    };
 
     // ---- per-bond quantities: tables (FP64 form), inline (mixed form) or streams
     int cmode = PM_STREAM, s0mode = pr->s0 ? PM_STREAM : PM_SCALAR, scmode = pr->sc ? PM_STREAM : PM_SCALAR;
     std::vector<double> tabc, tabp;
     std::vector<uint32_t> cidx, pidx;
-    if (!mixed) {
+    if (sp.ok) {
+        // key tables: c, then s0 and / or sc where they are functions of the key
+        cmode = PM_TABLE;
+        s0mode = sp.s0mode;
+        scmode = sp.scmode;
+        tabc = sp.tc;
+        tabp = sp.ts0;
+        tabp.insert(tabp.end(), sp.tsc.begin(), sp.tsc.end());
+    } else if (!mixed) {
         if (make_table(pr->BONDSTIFFNESS, nB, TAB_MAX, tabc, cidx)) cmode = PM_TABLE;
         // one per-bond parameter may use the second table
         if (cmode == PM_TABLE && (pr->s0 != nullptr) != (pr->sc != nullptr)) {
@@ -1000,8 +1427,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         if (pr->s0) s0mode = PM_INLINE;
         else if (pr->sc) scmode = PM_INLINE;
     }
-    std::vector<uint4> h_rec(nE);
-    std::memset(h_rec.data(), 0, sizeof(uint4) * nE);
+    std::vector<uint4> h_rec(sp.ok ? 0 : nE);
+    if (!sp.ok) std::memset(h_rec.data(), 0, sizeof(uint4) * nE);
     std::vector<double> h_c64, h_s064, h_sc64;
     std::vector<float> h_s032, h_sc032;
     if (!mixed && cmode == PM_STREAM) h_c64.assign(nE, 0.0);
@@ -1016,6 +1443,22 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     const double V = pr->cellVolume;
     for (int k = 0; k < nB; k++) {
         const int si = s->slot_of[bi[k]], sj = s->slot_of[bj[k]];
+        if (sp.ok) {
+            const long long ei = dense_at(si, sp.oi[k]), ej = dense_at(sj, sp.oj[k]);
+            s->entry_i[k] = ei;
+            s->entry_j[k] = ej;
+            for (long long e : {ei, ej}) {
+                if (s0mode == PM_STREAM) {
+                    if (mixed) h_s032[e] = (float)pr->s0[k];
+                    else h_s064[e] = pr->s0[k];
+                }
+                if (scmode == PM_STREAM) {
+                    if (mixed) h_sc032[e] = (float)pr->sc[k];
+                    else h_sc64[e] = pr->sc[k];
+                }
+            }
+            continue;
+        }
         const uint32_t type = pr->BONDTYPE ? (uint32_t)pr->BONDTYPE[k] : 0u;
         const uint32_t flags = (type << TYPE_SHIFT) | ((pr->fail && pr->fail[k] == 0) ? FAILED_BIT : 0u);
         const long long ei = entry_at(si, fillpos[si]++);
@@ -1055,7 +1498,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         }
     }
     // padding entries point at their own slot (their arithmetic is masked off)
-    for (int sl = 0; sl < s->nSlices; sl++)
+    for (int sl = 0; sl < (sp.ok ? 0 : s->nSlices); sl++)
         for (int t = 0; t < SLICE; t++) {
             int slot = sl * SLICE + t;
             long long w = (slice_off[sl + 1] - slice_off[sl]) / SLICE;
@@ -1087,7 +1530,20 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     uint4 *recd;
     if ((rc = dalloc(&rowlend, nSlots, s->owned))) return fail(rc);
     if ((rc = dalloc(&soffd, s->nSlices + 1, s->owned))) return fail(rc);
-    if ((rc = dalloc(&recd, nE, s->owned))) return fail(rc);
+    if (!sp.ok && (rc = dalloc(&recd, nE, s->owned))) return fail(rc);
+    // stencil layout
+    void *hsd = nullptr;
+    uint4 *maskd = nullptr;
+    int4 *stoffd = nullptr;
+    float4 *sxid = nullptr;
+    float *svcfd = nullptr, *tabfd = nullptr;
+    if (sp.ok) {
+        if ((rc = dalloc((char **)&hsd, (size_t)nE * (mixed ? 4 : 8), s->owned))) return fail(rc);
+        if ((rc = dalloc(&maskd, nSlots, s->owned))) return fail(rc);
+        if ((rc = dalloc(&stoffd, sp.noff, s->owned))) return fail(rc);
+        if ((rc = dalloc(&sxid, sp.noff, s->owned))) return fail(rc);
+        if ((rc = dalloc(&svcfd, sp.noff, s->owned))) return fail(rc);
+    }
     double *cstrd = nullptr, *tabd = nullptr;
     void *s0d = nullptr, *scd = nullptr;
     const size_t ew = mixed ? 4 : 8;
@@ -1096,6 +1552,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     if (scmode == PM_STREAM && (rc = dalloc((char **)&scd, nE * ew, s->owned))) return fail(rc);
     const size_t ntab = tabc.size() + tabp.size();
     if ((rc = dalloc(&tabd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
+    if (sp.ok && mixed && (rc = dalloc(&tabfd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
     double *partd;
     if ((rc = dalloc(&partd, 2 * (size_t)(nSlots / BLOCK) + 2, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->ds, 1, s->owned))) return fail(rc);
@@ -1127,6 +1584,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
                 uint64_t k1 = (uint64_t)std::floor(xv[1] / hlat[1] + 0.5);
                 uint64_t k2 = (uint64_t)std::floor(xv[2] / hlat[2] + 0.5);
                 hpos[q].q = k0 | (k1 << LAT_BITS) | (k2 << (2 * LAT_BITS));
+                if (sp.ok) hpos[q].q |= (uint64_t)sp.gidx[n] << G_SHIFT;
             } else {
                 hXr[q].x = xv[0];
                 hXr[q].y = xv[1];
@@ -1150,7 +1608,56 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     }
     BBPD_HIP(hipMemcpy(rowlend, rowlen.data(), sizeof(int) * nSlots, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy(soffd, slice_off.data(), sizeof(long long) * slice_off.size(), hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy(recd, h_rec.data(), sizeof(uint4) * nE, hipMemcpyHostToDevice));
+    if (!sp.ok) BBPD_HIP(hipMemcpy(recd, h_rec.data(), sizeof(uint4) * nE, hipMemcpyHostToDevice));
+    if (sp.ok) {
+        // per-slot masks; history 0, its sign bit set for bonds that start failed (brittle law)
+        std::vector<uint4> hm(nSlots, make_uint4(0, 0, 0, 0));
+        auto setbit = [&](int slot, int o) {
+            uint32_t *w = reinterpret_cast<uint32_t *>(&hm[slot]);
+            w[o >> 5] |= 1u << (o & 31);
+        };
+        for (int k = 0; k < nB; k++) {
+            setbit(s->slot_of[bi[k]], sp.oi[k]);
+            setbit(s->slot_of[bj[k]], sp.oj[k]);
+        }
+        BBPD_HIP(hipMemcpy(maskd, hm.data(), sizeof(uint4) * nSlots, hipMemcpyHostToDevice));
+        if (mixed) {
+            std::vector<float> h0(nE, 0.0f);
+            for (int k = 0; k < nB; k++)
+                if (pr->fail && pr->fail[k] == 0) h0[s->entry_i[k]] = h0[s->entry_j[k]] = -0.0f;
+            BBPD_HIP(hipMemcpy(hsd, h0.data(), sizeof(float) * nE, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> h0(nE, 0.0);
+            for (int k = 0; k < nB; k++)
+                if (pr->fail && pr->fail[k] == 0) h0[s->entry_i[k]] = h0[s->entry_j[k]] = -0.0;
+            BBPD_HIP(hipMemcpy(hsd, h0.data(), sizeof(double) * nE, hipMemcpyHostToDevice));
+        }
+        BBPD_HIP(hipMemcpy(stoffd, sp.off.data(), sizeof(int4) * sp.noff, hipMemcpyHostToDevice));
+        // mixed form per offset: xi = h * d rounded to FP32 (so the offset and its negation give
+        // exactly negated vectors), |xi|, and the volume correction factor of |h * d|
+        std::vector<float4> hx(sp.noff);
+        std::vector<float> hv(sp.noff);
+        for (int o = 0; o < sp.noff; o++) {
+            const double e0 = hlat[0] * sp.off[o].y, e1 = hlat[1] * sp.off[o].z, e2 = hlat[2] * sp.off[o].w;
+            const float x0 = (float)e0, x1 = (float)e1, x2 = (float)e2;
+            hx[o] = make_float4(x0, x1, x2, std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2))));
+            const double L0 = std::sqrt((e0 * e0 + e1 * e1) + e2 * e2);
+            double vcf;
+            if (L0 <= hmr) vcf = 1;
+            else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
+            else vcf = 0;
+            hv[o] = (float)vcf;
+        }
+        BBPD_HIP(hipMemcpy(sxid, hx.data(), sizeof(float4) * sp.noff, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(svcfd, hv.data(), sizeof(float) * sp.noff, hipMemcpyHostToDevice));
+        if (mixed) {
+            // c * V, then the parameter tables, in FP32
+            std::vector<float> tf(ntab, 0.0f);
+            for (size_t i = 0; i < tabc.size(); i++) tf[i] = (float)(tabc[i] * V);
+            for (size_t i = 0; i < tabp.size(); i++) tf[tabc.size() + i] = (float)tabp[i];
+            if (ntab) BBPD_HIP(hipMemcpy(tabfd, tf.data(), sizeof(float) * ntab, hipMemcpyHostToDevice));
+        }
+    }
     if (cstrd) BBPD_HIP(hipMemcpy(cstrd, h_c64.data(), 8 * nE, hipMemcpyHostToDevice));
     if (s0d) BBPD_HIP(hipMemcpy(s0d, mixed ? (void *)h_s032.data() : (void *)h_s064.data(), ew * nE, hipMemcpyHostToDevice));
     if (scd) BBPD_HIP(hipMemcpy(scd, mixed ? (void *)h_sc032.data() : (void *)h_sc64.data(), ew * nE, hipMemcpyHostToDevice));
@@ -1201,6 +1708,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.s0mode = s0mode;
     P.scmode = scmode;
     P.adr = pr->integrator == BBPD_INTEGRATOR_ADR ? 1 : 0;
+    P.law_linear = pr->law == BBPD_LAW_LINEAR ? 1 : 0;
     P.V = V;
     P.horizon = horizon;
     P.hmr = hmr;
@@ -1218,7 +1726,17 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.D = pr->DAMPING;
     P.partials = partd;
     P.ds = s->ds;
-    s->lds_bytes = mixed ? 0 : sizeof(double) * ntab;
+    P.noff = sp.ok ? sp.noff : 0;
+    P.kg = sp.G;
+    P.soff = stoffd;
+    P.sxi = sxid;
+    P.svcf = svcfd;
+    P.smask = maskd;
+    P.hs = hsd;
+    P.tabf = tabfd;
+    P.ts0 = (int)tabc.size();
+    P.tsc = (int)(tabc.size() + (sp.ok ? sp.ts0.size() : 0));
+    s->lds_bytes = sp.ok ? (mixed ? sizeof(float) : sizeof(double)) * ntab : (mixed ? 0 : sizeof(double) * ntab);
 
     // ---- contact bodies and observables
     StepEnd &S = s->S;
@@ -1384,6 +1902,34 @@ extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
 
 namespace {
 
+// stencil-layout history <-> host doubles (one per entry)
+int fetch_hs(bbpd_solver *s, std::vector<double> &hv)
+{
+    const long long nE = s->nPadded;
+    hv.assign(nE, 0.0);
+    if (s->precision == BBPD_PRECISION_MIXED) {
+        std::vector<float> f(nE);
+        BBPD_HIP(hipMemcpy(f.data(), s->P.hs, sizeof(float) * nE, hipMemcpyDeviceToHost));
+        for (long long e = 0; e < nE; e++) hv[e] = (double)f[e];
+    } else {
+        BBPD_HIP(hipMemcpy(hv.data(), s->P.hs, sizeof(double) * nE, hipMemcpyDeviceToHost));
+    }
+    return BBPD_OK;
+}
+
+int put_hs(bbpd_solver *s, const std::vector<double> &hv)
+{
+    const long long nE = s->nPadded;
+    if (s->precision == BBPD_PRECISION_MIXED) {
+        std::vector<float> f(nE);
+        for (long long e = 0; e < nE; e++) f[e] = (float)hv[e];
+        BBPD_HIP(hipMemcpy(s->P.hs, f.data(), sizeof(float) * nE, hipMemcpyHostToDevice));
+    } else {
+        BBPD_HIP(hipMemcpy(s->P.hs, hv.data(), sizeof(double) * nE, hipMemcpyHostToDevice));
+    }
+    return BBPD_OK;
+}
+
 // device nodal state <-> reference layout (nNodes x 3, column-major) on the host
 int fetch_soa(bbpd_solver *s, double *const *comps, double *dst)
 {
@@ -1489,6 +2035,24 @@ extern "C" int bbpd_solver_get_bonds(bbpd_solver *s, double *H, double *flag, do
     BBPD_HIP(hipStreamSynchronize(s->stream));
     const long long nE = s->nPadded;
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
+    if (s->stencil) {
+        std::vector<double> hv;
+        int rc = fetch_hs(s, hv);
+        if (rc) return rc;
+        const bool linear = s->law == BBPD_LAW_LINEAR;
+        for (int k = 0; k < s->nB; k++) {
+            const double a = hv[s->entry_i[k]], b = hv[s->entry_j[k]];
+            if (!bits_equal(a, b)) {
+                bbpd::set_error("internal: the two ends of bond " + std::to_string(k + 1) + " disagree");
+                return BBPD_ERR_UNSUPPORTED;
+            }
+            const bool sgn = std::signbit(a);
+            if (H) H[k] = std::fabs(a);
+            if (flag) flag[k] = (!linear && sgn) ? 1.0 : 0.0;
+            if (failout) failout[k] = (linear && sgn) ? 0.0 : 1.0;
+        }
+        return BBPD_OK;
+    }
     std::vector<uint4> r(nE);
     BBPD_HIP(hipMemcpy(r.data(), s->P.rec, sizeof(uint4) * nE, hipMemcpyDeviceToHost));
     auto hist_of = [&](long long e) {
@@ -1563,7 +2127,28 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
                 bbpd::set_error("bondSofteningFactor must lie in [0, 1]");
                 return BBPD_ERR_ARG;
             }
-    if (H || flag || fail) {
+    if (s->stencil && (H || flag || fail)) {
+        const bool linear = s->law == BBPD_LAW_LINEAR;
+        if (fail && !linear)
+            for (int k = 0; k < s->nB; k++)
+                if (fail[k] != 1) {
+                    bbpd::set_error("fail other than 1 with a softening law needs the record layout "
+                                    "(BBPD_LAYOUT=records)");
+                    return BBPD_ERR_UNSUPPORTED;
+                }
+        std::vector<double> hv;
+        if ((rc = fetch_hs(s, hv))) return rc;
+        for (int k = 0; k < s->nB; k++) {
+            const double cur = hv[s->entry_i[k]];
+            const double mag = H ? H[k] : std::fabs(cur);
+            bool sgn = std::signbit(cur);
+            if (!linear && flag) sgn = flag[k] == 1;
+            if (linear && fail) sgn = fail[k] == 0;
+            const double v = sgn ? -mag : mag;
+            hv[s->entry_i[k]] = hv[s->entry_j[k]] = v;
+        }
+        if ((rc = put_hs(s, hv))) return rc;
+    } else if (H || flag || fail) {
         std::vector<uint4> r(nE);
         BBPD_HIP(hipMemcpy(r.data(), s->P.rec, sizeof(uint4) * nE, hipMemcpyDeviceToHost));
         for (int k = 0; k < s->nB; k++) {
@@ -1624,8 +2209,8 @@ extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *
     const int64_t w = mixed ? 4 : 8;
     // algorithmic bytes of one bond-kernel launch as laid out here: per entry the 16-byte record
     // plus any streamed per-bond quantity; per node the state the kernel reads and writes
-    int64_t per_entry = 16 + (s->P.cmode == PM_STREAM && !mixed ? 8 : 0) + (s->P.s0mode == PM_STREAM ? w : 0) +
-                        (s->P.scmode == PM_STREAM ? w : 0);
+    int64_t per_entry = (s->stencil ? w : 16) + (s->P.cmode == PM_STREAM && !mixed ? 8 : 0) +
+                        (s->P.s0mode == PM_STREAM ? w : 0) + (s->P.scmode == PM_STREAM ? w : 0);
     int64_t per_node;
     if (s->integrator == BBPD_INTEGRATOR_ADR)
         per_node = 32 /*pos r*/ + 24 /*X*/ + 24 /*F w*/ + 24 /*Fp*/ + 24 /*v*/ + (mixed ? 0 : 24) /*u*/ + 8 /*m*/ +
@@ -1634,11 +2219,18 @@ extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *
         per_node = 32 /*pos r*/ + 24 /*X*/ + 24 /*pos w*/ + 48 /*v rw*/ + (mixed ? 0 : 48) /*u rw*/ + 24 /*F w*/ +
                    8 /*rho*/ + 1 /*cons*/ + 4 /*rowlen*/;
     if (!s->lattice) per_node += 32;
+    // stencil layout: a 16-byte bond mask instead of X and the row length
+    if (s->stencil) per_node += 16 - 24 - 4;
     if (device_bytes) *device_bytes = (int64_t)s->device_bytes;
     if (bond_bytes) *bond_bytes = per_entry * s->nEntries + per_node * s->nN;
     if (n_entries) *n_entries = s->nEntries;
     if (n_padded) *n_padded = s->nPadded;
     return BBPD_OK;
+}
+
+extern "C" const char *bbpd_solver_layout(bbpd_solver *s)
+{
+    return s ? s->layout.c_str() : "";
 }
 
 extern "C" int bbpd_solver_set_timing(bbpd_solver *s, int on)

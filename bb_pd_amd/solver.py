@@ -118,6 +118,10 @@ class Solver:
         return dict(reaction=r[:, 0].copy(), reaction_static=r[:, 1].copy(), deflection=r[:, 2].copy(),
                     cmod=r[:, 3].copy())
 
+    def layout(self) -> str:
+        """"stencil" or "records (<reason>)": the bond-table layout chosen at set-up."""
+        return self._L.bbpd_solver_layout(self._h).decode()
+
     def info(self) -> dict:
         v = [C.c_int64() for _ in range(4)]
         _lib.check(self._L.bbpd_solver_info(self._h, *[C.byref(x) for x in v]), "bbpd_solver_info")

@@ -139,9 +139,9 @@ def precision_study(case, fp64, mixed, ipi):
     reaction, calculatereactionforceFast.m:3-5).
 
     * free-running: peak of the increment-averaged load curve of each form;
-    * rounding floor: the FP64 run restarted after increment 1 with one coordinate moved by one ulp
-      and run to the end; its peak difference is what rounding alone does to this (unconverged) ADR
-      load history;
+    * rounding floors: the FP64 run restarted after increment 1 with one coordinate moved by one ulp,
+      and with every coordinate moved by 2^-24 relative (the size of FP32 rounding), each run to the
+      end; their peak differences are what such perturbations alone do to this load history;
     * same state: from FP64 states along the history, incl. the FP64 peak increment, one step of each
       form; errors of the load, nodal forces, flags, broken bonds and damage field."""
     A, B = fp64["solver"], mixed["solver"]
@@ -188,6 +188,23 @@ def precision_study(case, fp64, mixed, ipi):
     R2 = np.concatenate([fp64["hist"]["reaction_static"][:st["step"]], A.history()["reaction_static"]])
     c2 = increment_curve(R2, ipi)
     p2 = int(np.argmax(c2))
+    # perturbation of FP32 rounding size: every coordinate moved by 2^-24 relative, random sign
+    st = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in ck[ipi].items()}
+    sgn = np.where(np.random.default_rng(7).random(st["x"].shape) < 0.5, -1.0, 1.0)
+    st["x"] = np.asfortranarray(st["x"] * (1 + sgn * 2.0 ** -24))
+    st["u"] = np.asfortranarray(st["x"] - case.X)
+    A.history()
+    A.restore(st["step"], **{k: v for k, v in st.items() if k != "step"})
+    left = total - st["step"]
+    while left > 0:
+        n = min(10 * ipi, left)
+        A.step(n)
+        A.sync()
+        left -= n
+        log(f"    FP32-sized perturbation run: {total - left} steps")
+    R3 = np.concatenate([fp64["hist"]["reaction_static"][:st["step"]], A.history()["reaction_static"]])
+    c3 = increment_curve(R3, ipi)
+    p3 = int(np.argmax(c3))
     peak = c64[p64]
     at_peak = next(x for x in same if x["step"] == peak_step)
     return dict(
@@ -197,8 +214,11 @@ def precision_study(case, fp64, mixed, ipi):
         peak_fp64=float(peak), peak_increment_fp64=p64 + 1, peak_mixed=float(cm[pm]), peak_increment_mixed=pm + 1,
         peak_load_err=float((cm[pm] - peak) / peak),
         fp64_rounding_floor=float((c2[p2] - peak) / peak),
+        fp64_fp32_sized_floor=float((c3[p3] - peak) / peak),
         curve_err_max=float(np.abs(cm - c64).max() / peak),
         curve_floor_max=float(np.abs(c2 - c64).max() / peak),
+        curve_fp32_sized_floor_max=float(np.abs(c3 - c64).max() / peak),
+        curve_fp64_ulp=[round(float(v), 3) for v in c2], curve_fp64_fp32_sized=[round(float(v), 3) for v in c3],
         same_state_peak_load_err=at_peak["load_err"],
         same_state=same)
 
@@ -329,7 +349,8 @@ def main():
         prec = precision_study(case, fp64, mixed, ipi)
         log(f"precision study {time.time() - t0:.0f}s: peak fp64 {prec['peak_fp64']:.1f} (inc "
             f"{prec['peak_increment_fp64']}), mixed {prec['peak_mixed']:.1f}, err {prec['peak_load_err']:.2e}, "
-            f"fp64 rounding floor {prec['fp64_rounding_floor']:.2e}, same-state peak err "
+            f"fp64 rounding floor {prec['fp64_rounding_floor']:.2e}, FP32-sized "
+            f"{prec['fp64_fp32_sized_floor']:.2e}, same-state peak err "
             f"{prec['same_state_peak_load_err']:.2e}")
     fp64["solver"].close()
     mixed["solver"].close()

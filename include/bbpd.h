@@ -239,6 +239,11 @@ int bbpd_solver_history(bbpd_solver *s, double *rows, int cap, int *n_out);
 /* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */
 int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel_bytes, int64_t *n_entries,
                      int64_t *n_padded_entries);
+/* the bond-table layout chosen at set-up: "stencil" (full lattice box, rows in lattice-offset
+ * order: neighbours implicit, per entry only the history) or "records (<why not stencil>)".  The
+ * string lives as long as the solver.  Default: stencil where it applies for the FP32-mixed form,
+ * records for plain FP64; BBPD_LAYOUT=stencil or BBPD_LAYOUT=records in the environment chooses. */
+const char *bbpd_solver_layout(bbpd_solver *s);
 /* average duration (ms) of the bond kernel over the steps since the last call, from HIP events
  * recorded on the solver's stream around each launch (0 when timing is off) */
 int bbpd_solver_set_timing(bbpd_solver *s, int on);

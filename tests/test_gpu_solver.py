@@ -293,3 +293,113 @@ def test_mixed_precision_error_report(gpu):
         assert err_u < 2e-3
         assert flips <= max(5, 0.02 * fla.sum())
         assert abs(broken_a - broken_b) <= max(5, 0.02 * broken_a)
+
+
+def _solver_with_layout(case, precision, layout):
+    import os
+    old = os.environ.get("BBPD_LAYOUT")
+    try:
+        os.environ["BBPD_LAYOUT"] = layout
+        return Solver(case, precision)
+    finally:
+        if old is None:
+            os.environ.pop("BBPD_LAYOUT", None)
+        else:
+            os.environ["BBPD_LAYOUT"] = old
+
+
+@pytest.mark.parametrize("law", [_lib.LAW_BILINEAR, _lib.LAW_TRILINEAR, _lib.LAW_LINEAR, _lib.LAW_EXPONENTIAL])
+def test_stencil_layout_equals_records(gpu, law):
+    """The stencil layout (implicit neighbours, history only per entry) and the record layout give
+    bit-identical FP64 trajectories: u, v, x, F, H, flags, fail, reaction history."""
+    case = ec_case(law)
+    a = _solver_with_layout(case, _lib.FP64, "stencil")
+    b = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b:
+        assert a.layout() == "stencil", a.layout()
+        assert b.layout().startswith("records"), b.layout()
+        a.step(250)
+        b.step(250)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+        assert same(a.history()["reaction"], b.history()["reaction"])
+        oa, ob = a.observables(), b.observables()
+        for k in ("n_softening", "n_broken", "n_failed"):
+            assert oa[k] == ob[k], k
+        if law == _lib.LAW_LINEAR:
+            assert oa["n_failed"] > 0
+        else:
+            assert oa["n_softening"] > 0
+
+
+def test_stencil_layout_restore_and_adr(gpu):
+    """A checkpoint of the stencil layout restores into the record layout (the history's sign-bit
+    flag round-trips) and the two continue bit for bit under Euler-Cromer.  Under ADR the damping
+    sum is a block tree over slots, whose order differs between the layouts, so one step from a
+    shared state agrees to 1e-12."""
+    case = ec_case(_lib.LAW_BILINEAR)
+    a = _solver_with_layout(case, _lib.FP64, "stencil")
+    b = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b:
+        assert a.layout() == "stencil"
+        a.step(150)
+        ca = a.checkpoint()
+        assert ca["flag"].sum() > 0
+        step = ca.pop("step")
+        b.restore(step, **ca)
+        a.step(100)
+        b.step(100)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+    case = adr_case()
+    a = _solver_with_layout(case, _lib.FP64, "stencil")
+    b = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b:
+        assert a.layout() == "stencil"
+        a.step(450)
+        ca = a.checkpoint()
+        step = ca.pop("step")
+        b.restore(step, **ca)
+        a.step(1)
+        b.step(1)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max()
+        assert a.observables()["cn"] == pytest.approx(b.observables()["cn"], rel=1e-12, abs=1e-14)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+
+
+def test_stencil_layout_mixed_close_to_records(gpu):
+    """Mixed form: the two layouts round c*V*vcf and xi differently (FP32), nothing else."""
+    case = ec_case(_lib.LAW_BILINEAR, steps=300)
+    a = _solver_with_layout(case, _lib.MIXED, "stencil")
+    b = _solver_with_layout(case, _lib.MIXED, "records")
+    with a, b:
+        assert a.layout() == "stencil"
+        a.step(300)
+        b.step(300)
+        ua, ub = a.nodal()[0], b.nodal()[0]
+        assert np.abs(ua - ub).max() <= 1e-5 * np.abs(ub).max()
+        fa, fb = a.bonds()[1], b.bonds()[1]
+        assert abs(int(fa.sum()) - int(fb.sum())) <= max(3, 0.01 * fb.sum())
+
+
+def test_layout_defaults_and_fallback(gpu):
+    """Defaults: stencil for FP32-mixed, records for FP64.  A bond list not sorted by node id takes
+    the record layout even when the stencil one is asked for."""
+    case = ec_case(_lib.LAW_BILINEAR)
+    import os
+    assert "BBPD_LAYOUT" not in os.environ
+    with Solver(case, _lib.MIXED) as a, Solver(case, _lib.FP64) as b:
+        assert a.layout() == "stencil"
+        assert b.layout() == "records (FP64 default)"
+    order = np.random.default_rng(1).permutation(case.nBonds)
+    case.BONDLIST = np.asfortranarray(case.BONDLIST[order])
+    for name in ("L0", "vcf", "c", "BONDTYPE", "fail", "s0"):
+        setattr(case, name, getattr(case, name)[order])
+    with _solver_with_layout(case, _lib.MIXED, "stencil") as s:
+        assert s.layout().startswith("records (rows not in ascending"), s.layout()
