@@ -67,13 +67,42 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // how a per-bond quantity reaches the kernel
 enum ParamMode { PM_SCALAR = 0, PM_TABLE = 1, PM_STREAM = 2, PM_INLINE = 3 };
 
-struct DevScalars {
-    long long step;    // steps completed
-    unsigned int done; // blocks of the step-end kernel finished this step
-    double cn;
-    double cn1, cn2;
+// A region is one problem instance: a block-aligned range of slots with its own stencil, tables,
+// law scalars, contact bodies, load schedule, history and step counter.  A single problem is one
+// region; a batch packs many into one launch of each kernel.
+struct Region {
+    int base, nN, nSlots;          // slots [base, base + nSlots); the first nN hold nodes
+    int blk0, nblk;                // global blocks [blk0, blk0 + nblk)
+    long long e0;                  // stencil layout: first entry of the region
+    int noff, off0;                // stencil layout: offsets off0 .. off0 + noff - 1
+    int kg, tab0, ntab, ts0, tsc;  // stencil layout: key tables (ts0 / tsc relative to tab0)
+    double s0, s1, sc;             // scalar law parameters
+    int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
+    int schedule, nTimeSteps, ipi; // load schedule
+    double applied, increment;
+    int ref_slot, cmod1, cmod2;    // observation slots, -1 = none
+    long long hist0;               // history ring rows hist0 .. hist0 + hist_cap - 1
+    int hist_cap;
+    double stop_frac, stop_disp;   // finish when the load falls below stop_frac of its peak after
+                                   // the peak, or the load point passes stop_disp (0 = never)
+};
+
+struct RegionState {
+    long long step;     // steps completed
+    unsigned int done;  // step-end blocks of this region finished this step
+    double cn, cn1, cn2;
     double reaction[BBPD_MAX_BODIES];
     double reaction_static;
+    double peak;        // largest static reaction at an increment end
+    int finished;       // the stop criterion was met (the region keeps stepping until retired)
+};
+
+struct Body {
+    const int *fam;     // member slots (absolute)
+    int nFam;
+    double cx, cz, R;
+    int moves;          // follows the region's load schedule
+    int region, index;  // owner and index within the region (body 0 gives the reaction)
 };
 
 struct __align__(32) NodeRec {
@@ -83,7 +112,7 @@ struct __align__(32) NodeRec {
 
 struct Params {
     // nodes
-    int nN, nSlots;
+    int nSlots;
     const double *X0, *X1, *X2;
     NodeRec *pos[2];
     const NodeRec *Xrec; // undeformed coordinates as records (off-lattice inputs)
@@ -105,22 +134,31 @@ struct Params {
     int cmode, s0mode, scmode;
     int adr;
     int law_linear; // the brittle latch: the stencil layout's sign bit is the failed flag
-    // scalars
+    // scalars shared by every region
     double V, horizon, hmr, hpr, two_r; // hmr = horizon - RADIJ, hpr = horizon + RADIJ, two_r = 2*RADIJ
-    double s0, s1, sc, k, alpha, denom_exp, sc_steel, ff;
+    double k, alpha, denom_exp, sc_steel, ff;
     double DT, D;
-    double *partials;
-    DevScalars *ds;
+    double *partials;     // per global block: the ADR Rayleigh partial sums
+    double *hist;         // history rows (N_HIST doubles each), regions own ranges of rows
+    const int *active;    // active regions (one k_adr_cn block each)
+    // regions
+    const Region *reg;
+    RegionState *rst;
+    const int *blkreg;    // global block -> region
+    const int *grid;      // launch block (after the XCD remap) -> global block, -1 = idle
+    const Body *bodies;
+    const int2 *se;       // step-end blocks: (region, body index or -1)
+    int nse;
+    long long *tick;      // launches completed: the position-buffer parity
+    unsigned int *tick_done; // step-end blocks finished this launch
     // stencil layout (full lattice box, rows in lattice-offset order): neighbours are implicit
-    int noff;             // offsets, padded to even
+    int stencil;
     const int4 *soff;     // per offset: slot delta, dx, dy, dz
     const float4 *sxi;    // per offset (mixed form): xi = h * d in FP32, |xi|
     const float *svcf;    // per offset (mixed form): volume correction factor of |h * d|
     const uint4 *smask;   // per slot: bit o set when the node has the bond at offset o
     void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
     const float *tabf;    // mixed-form key tables (c * V, then parameter tables)
-    int ts0, tsc;         // start of the s0 / sc key table inside the tables
-    int kg;               // key = gidx_n * kg + gidx_m
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -197,11 +235,10 @@ __device__ inline uint32_t *rec_word(uint4 *rec, long long e, int word)
 // writing its next position record, or, for ADR, F and the block's Rayleigh-quotient partial sums
 // (calculatedampingcoefficient.m:61-85).  Every thread of the block calls it (block reduction).
 template <bool MIXED>
-__device__ __forceinline__ void finish_node(const Params &P, int n, int b, int cur, double f0, double f1, double f2,
-                                            double qn0, double qn1, double qn2, double Xn0, double Xn1, double Xn2,
-                                            double *sh)
+__device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool real, int cur, double f0, double f1,
+                                            double f2, double qn0, double qn1, double qn2, double Xn0, double Xn1,
+                                            double Xn2, double *sh)
 {
-    const bool real = n < P.nN;
     if (!P.adr) {
         if (!real) return;
         const int nxt = cur ^ 1;
@@ -264,10 +301,16 @@ __device__ __forceinline__ void finish_node(const Params &P, int n, int b, int c
         const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
         const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
         if (threadIdx.x == 0) {
-            P.partials[2 * b] = s1;
-            P.partials[2 * b + 1] = s2;
+            P.partials[2 * gb] = s1;
+            P.partials[2 * gb + 1] = s2;
         }
     }
+}
+
+// the global block a launch block works on (after the XCD remap), -1 = idle
+__device__ inline int launch_block(const Params &P)
+{
+    return P.grid[remap_block(blockIdx.x, gridDim.x)];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -304,9 +347,13 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
     using T = typename std::conditional<MIXED, float, double>::type;
     __shared__ double sh[BLOCK];
     extern __shared__ double tabs[]; // FP64 parameter tables
-    const int b = remap_block(blockIdx.x, gridDim.x);
-    const int n = b * BLOCK + threadIdx.x;
-    const int cur = (int)(P.ds->step & 1);
+    const int gb = launch_block(P);
+    if (gb < 0) return;
+    const int n = gb * BLOCK + threadIdx.x;
+    const Region &RG = P.reg[P.blkreg[gb]];
+    const bool real = n - RG.base < RG.nN;
+    const double rs0 = RG.s0, rs1 = RG.s1, rsc = RG.sc;
+    const int cur = (int)(*P.tick & 1);
     const NodeRec *__restrict__ pos = P.pos[cur];
     uint4 *__restrict__ rec = P.rec;
     if (!MIXED) {
@@ -359,8 +406,8 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             double H = __hiloint2double((int)d.r.w, (int)d.r.z);
             const double H0 = H;
             const double c = P.cmode == PM_TABLE ? tabc[d.r.y & 0xFFFFu] : d.c;
-            const double s0 = P.s0mode == PM_TABLE ? tabp[d.r.y >> 16] : (P.s0mode == PM_STREAM ? d.p0 : P.s0);
-            const double sc = P.scmode == PM_TABLE ? tabp[d.r.y >> 16] : (P.scmode == PM_STREAM ? d.p1 : P.sc);
+            const double s0 = P.s0mode == PM_TABLE ? tabp[d.r.y >> 16] : (P.s0mode == PM_STREAM ? d.p0 : rs0);
+            const double sc = P.scmode == PM_TABLE ? tabp[d.r.y >> 16] : (P.scmode == PM_STREAM ? d.p1 : rsc);
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n): exact negation of the
             // reference's x_j - x_i at the j end, so |.|, s and the force magnitude are identical
             const double dx = d.nb.x - qn0, dy = d.nb.y - qn1, dz = d.nb.z - qn2;
@@ -368,7 +415,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
             const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
             const double s = (l - L0) / L0;
-            apply_law<LAW, double>(s, s0, sc, P.s1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
+            apply_law<LAW, double>(s, s0, sc, rs1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
             // calculatevolumecorrectionfactors.m:20-32
             double vcf;
             if (L0 <= P.hmr) vcf = 1;
@@ -399,9 +446,9 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             const float H0 = H;
             const float kap = __uint_as_float(d.r.w);
             const float s0 = P.s0mode == PM_INLINE ? __uint_as_float(d.r.y)
-                                                   : (P.s0mode == PM_STREAM ? d.p0 : (float)P.s0);
+                                                   : (P.s0mode == PM_STREAM ? d.p0 : (float)rs0);
             const float sc = P.scmode == PM_INLINE ? __uint_as_float(d.r.y)
-                                                   : (P.scmode == PM_STREAM ? d.p1 : (float)P.sc);
+                                                   : (P.scmode == PM_STREAM ? d.p1 : (float)rsc);
             const float xi0 = (float)(Xm0 - Xn0), xi1 = (float)(Xm1 - Xn1), xi2 = (float)(Xm2 - Xn2);
             const float et0 = (float)(d.nb.x - qn0), et1 = (float)(d.nb.y - qn1), et2 = (float)(d.nb.z - qn2);
             const float y0 = xi0 + et0, y1 = xi1 + et1, y2 = xi2 + et2;
@@ -411,7 +458,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
                 fmaf(2.0f, fmaf(xi0, et0, fmaf(xi1, et1, xi2 * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
             const float rl = __builtin_amdgcn_rcpf(l);
             const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
-            apply_law<LAW, float>(s, s0, sc, (float)P.s1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
+            apply_law<LAW, float>(s, s0, sc, (float)rs1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
                                   (float)P.sc_steel, (float)P.ff, H, w);
             const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
             const float p = fail * kap * (1.0f - H) * s * rl;
@@ -440,7 +487,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             iB = iB2;
         }
     }
-    finish_node<MIXED>(P, n, b, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
+    finish_node<MIXED>(P, n, gb, real, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -473,22 +520,28 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
     __shared__ float4 s_xi[MIXED ? ST_MAX : 1];
     __shared__ float s_vcf[MIXED ? ST_MAX : 1];
     extern __shared__ double tabs[]; // key tables (FP64, or FP32 in the mixed form)
-    const int b = remap_block(blockIdx.x, gridDim.x);
-    const int n = b * BLOCK + threadIdx.x;
-    const int cur = (int)(P.ds->step & 1);
+    const int gb = launch_block(P);
+    if (gb < 0) return;
+    const int n = gb * BLOCK + threadIdx.x;
+    const Region &RG = P.reg[P.blkreg[gb]];
+    const int base = RG.base, noff = RG.noff, off0 = RG.off0, kg = RG.kg, tab0 = RG.tab0, ntab = RG.ntab;
+    const int ts0 = RG.ts0, tsc = RG.tsc;
+    const long long e0 = RG.e0;
+    const bool real = n - base < RG.nN;
+    const double rs0 = RG.s0, rs1 = RG.s1, rsc = RG.sc;
+    const int cur = (int)(*P.tick & 1);
     const NodeRec *__restrict__ pos = P.pos[cur];
-    const int ntab = P.ntabc + P.ntabp;
     if constexpr (MIXED) {
         float *tf = reinterpret_cast<float *>(tabs);
-        for (int i = threadIdx.x; i < ntab; i += BLOCK) tf[i] = P.tabf[i];
+        for (int i = threadIdx.x; i < ntab; i += BLOCK) tf[i] = P.tabf[tab0 + i];
     } else {
-        for (int i = threadIdx.x; i < ntab; i += BLOCK) tabs[i] = P.tab[i];
+        for (int i = threadIdx.x; i < ntab; i += BLOCK) tabs[i] = P.tab[tab0 + i];
     }
-    for (int i = threadIdx.x; i < P.noff; i += BLOCK) {
-        s_off[i] = P.soff[i];
+    for (int i = threadIdx.x; i < noff; i += BLOCK) {
+        s_off[i] = P.soff[off0 + i];
         if constexpr (MIXED) {
-            s_xi[i] = P.sxi[i];
-            s_vcf[i] = P.svcf[i];
+            s_xi[i] = P.sxi[off0 + i];
+            s_vcf[i] = P.svcf[off0 + i];
         }
     }
     __syncthreads();
@@ -506,8 +559,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
     unsigned __int128 mk = ((unsigned __int128)(((uint64_t)msk.w << 32) | msk.z) << 64) |
                            (((uint64_t)msk.y << 32) | msk.x);
     T *__restrict__ hs = reinterpret_cast<T *>(P.hs);
-    const int noff = P.noff;
-    const long long ebase = (long long)(n >> 6) * noff * SLICE + (n & (SLICE - 1));
+    const long long ebase = e0 + (long long)((n - base) >> 6) * noff * SLICE + (n & (SLICE - 1));
     double f0 = 0, f1 = 0, f2 = 0;
     // `bit` = this lane has the bond at offset o
     auto load = [&](StEnt<T> &d, int o, bool bit) {
@@ -522,15 +574,15 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
     auto compute = [&](const StEnt<T> &d, int o, bool bit) {
         if (!bit) return;
         const long long e = ebase + (long long)o * SLICE;
-        const int key = gme * P.kg + (int)(d.nb.q >> G_SHIFT);
+        const int key = gme * kg + (int)(d.nb.q >> G_SHIFT);
         const bool sgn = MIXED ? (__float_as_uint((float)d.h) >> 31) != 0u
                                : (__double_as_longlong((double)d.h) < 0);
         const uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
         uint32_t w = sgn ? mark : 0u;
         T H = d.h < T(0) ? -d.h : d.h; // |h|, and -0 -> +0
         if (H == T(0)) H = T(0);
-        const T s0 = P.s0mode == PM_TABLE ? tab[P.ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : (T)P.s0);
-        const T sc = P.scmode == PM_TABLE ? tab[P.tsc + key] : (P.scmode == PM_STREAM ? d.p1 : (T)P.sc);
+        const T s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : (T)rs0);
+        const T sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : (T)rsc);
         if constexpr (!MIXED) {
             const int4 so = s_off[o];
             const double Xm0 = P.hx * (double)(kx + so.y), Xm1 = P.hy * (double)(ky + so.z),
@@ -542,7 +594,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
             const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
             const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
             const double s = (l - L0) / L0;
-            apply_law<LAW, double>(s, s0, sc, P.s1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
+            apply_law<LAW, double>(s, s0, sc, rs1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
             double vcf; // calculatevolumecorrectionfactors.m:20-32
             if (L0 <= P.hmr) vcf = 1;
             else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
@@ -575,7 +627,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
                 fmaf(2.0f, fmaf(xi.x, et0, fmaf(xi.y, et1, xi.z * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
             const float rl = __builtin_amdgcn_rcpf(l);
             const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
-            apply_law<LAW, float>(s, s0, sc, (float)P.s1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
+            apply_law<LAW, float>(s, s0, sc, (float)rs1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
                                   (float)P.sc_steel, (float)P.ff, H, w);
             const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
             const float p = fail * kap * (1.0f - H) * s * rl;
@@ -602,17 +654,20 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
             mk >>= 2;
         }
     }
-    finish_node<MIXED>(P, n, b, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
+    finish_node<MIXED>(P, n, gb, real, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
 }
 
-// ADR: damping coefficient from the block partials (calculatedampingcoefficient.m:90-112)
-__global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P, int nPart)
+// ADR: damping coefficient of each active region from its blocks' partials
+// (calculatedampingcoefficient.m:90-112); one block per region, a fixed summation order
+__global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P)
 {
     __shared__ double sh[BLOCK];
+    const int ri = P.active[blockIdx.x];
+    const Region &RG = P.reg[ri];
     double c1 = 0, c2 = 0;
-    for (int q = threadIdx.x; q < nPart; q += BLOCK) {
-        c1 = c1 + P.partials[2 * q];
-        c2 = c2 + P.partials[2 * q + 1];
+    for (int q = threadIdx.x; q < RG.nblk; q += BLOCK) {
+        c1 = c1 + P.partials[2 * (RG.blk0 + q)];
+        c2 = c2 + P.partials[2 * (RG.blk0 + q) + 1];
     }
     const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
     const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
@@ -620,9 +675,10 @@ __global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P, int nPart)
         double cn = 0;
         if (s2 != 0) cn = ((s1 / s2) > 0) ? 2 * sqrt(s1 / s2) : 0;
         if (cn > 2) cn = 1.9;
-        P.ds->cn = cn;
-        P.ds->cn1 = s1;
-        P.ds->cn2 = s2;
+        RegionState &st = P.rst[ri];
+        st.cn = cn;
+        st.cn1 = s1;
+        st.cn2 = s2;
     }
 }
 
@@ -631,11 +687,14 @@ __global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P, int nPart)
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
 {
-    const int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= P.nN) return;
-    const int cur = (int)(P.ds->step & 1), nxt = cur ^ 1;
-    const int it = (int)(P.ds->step + 1);
-    const double cn = P.ds->cn, DT = P.DT;
+    const int gb = launch_block(P);
+    if (gb < 0) return;
+    const int n = gb * BLOCK + threadIdx.x;
+    const int ri = P.blkreg[gb];
+    if (n - P.reg[ri].base >= P.reg[ri].nN) return;
+    const int cur = (int)(*P.tick & 1), nxt = cur ^ 1;
+    const long long it = P.rst[ri].step + 1;
+    const double cn = P.rst[ri].cn, DT = P.DT;
     const double minv = 1 / P.mass[n];
     const unsigned char cf = P.cons[n];
     double *vv[3] = {P.v0, P.v1, P.v2};
@@ -675,58 +734,51 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
 
 // ------------------------------------------------------------------------------------------------
 // Step end: contact with every rigid body (calculatecontactforce.m:62-121), the load schedule
-// (smoothstepdata.m:40-42 or a staircase of increments), the step's observables and the step counter.
+// (smoothstepdata.m:40-42 or a staircase of increments), the step's observables and step counters.
+// One block per (region, body); a region without bodies still gets one block for its observables.
 // ------------------------------------------------------------------------------------------------
-struct StepEnd {
-    int nBodies;
-    const int *fam[BBPD_MAX_BODIES];
-    int nFam[BBPD_MAX_BODIES];
-    double cx[BBPD_MAX_BODIES], cz[BBPD_MAX_BODIES], R[BBPD_MAX_BODIES];
-    int moves[BBPD_MAX_BODIES];
-    int schedule, nTimeSteps, ipi;
-    double applied, increment;
-    int ref_slot, cmod1, cmod2; // -1 = none
-    double *hist;               // HIST_CAP x N_HIST ring
-};
-
-__device__ inline double schedule_disp(const StepEnd &S, long long i)
+__device__ inline double schedule_disp(const Region &R, long long i)
 {
-    if (S.schedule == 0) {
+    if (R.schedule == 0) {
         // smoothstepdata(i, 1, N, 0, a): total displacement at step i
-        double xi = ((double)i - 1) / ((double)S.nTimeSteps - 1);
-        return 0 + (S.applied - 0) * (xi * xi * xi) * (10 - 15 * xi + 6 * (xi * xi));
+        double xi = ((double)i - 1) / ((double)R.nTimeSteps - 1);
+        return 0 + (R.applied - 0) * (xi * xi * xi) * (10 - 15 * xi + 6 * (xi * xi));
     }
-    long long inc = (i + S.ipi - 1) / S.ipi;
-    return S.increment * (double)inc;
+    long long inc = (i + R.ipi - 1) / R.ipi;
+    return R.increment * (double)inc;
 }
 
 template <bool MIXED>
-__global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
+__global__ __launch_bounds__(1024) void k_step_end(Params P)
 {
-    // one block per contact body (families are disjoint); the last block to finish records the
-    // step's observables and advances the step counter
+    // the last block of a region records the region's observables and advances its step counter;
+    // the last block of the launch advances the position-buffer parity
     __shared__ double vals[1024];
     __shared__ int wofs[17];
-    __shared__ int last_block;
-    const int nxt = (int)((P.ds->step + 1) & 1);
-    const long long it = P.ds->step + 1;
+    __shared__ int last_block, last_all;
+    const int2 job = P.se[blockIdx.x];
+    const int ri = job.x, bi = job.y;
+    const Region &RG = P.reg[ri];
+    RegionState &RS = P.rst[ri];
+    const int nxt = (int)((*P.tick + 1) & 1);
+    const long long it = RS.step + 1;
     NodeRec *pos = P.pos[nxt];
     double *uu[3] = {P.u0, P.u1, P.u2};
     double *vv[3] = {P.v0, P.v1, P.v2};
     const double *XX[3] = {P.X0, P.X1, P.X2};
-    const int bdy = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double total = 0, rs = 0;
-    if (bdy < S.nBodies) {
-        const double disp = schedule_disp(S, it);
-        const double cx = S.cx[bdy], czm = S.cz[bdy] + (S.moves[bdy] ? disp : 0.0), R = S.R[bdy];
-        const int nf = S.nFam[bdy];
+    if (bi >= 0) {
+        const Body &B = P.bodies[RG.body0 + bi];
+        const double disp = schedule_disp(RG, it);
+        const double cx = B.cx, czm = B.cz + (B.moves ? disp : 0.0), R = B.R;
+        const int nf = B.nFam;
         for (int base = 0; base < nf; base += blockDim.x) {
             const int t = base + threadIdx.x;
             bool hit = false;
             double d_z = 0;
             if (t < nf) {
-                const int i = S.fam[bdy][t];
+                const int i = B.fam[t];
                 NodeRec pr = pos[i];
                 double pv[3] = {pr.x, pr.y, pr.z};
                 double xc[3];
@@ -774,24 +826,25 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
                 for (int q = 0; q < wofs[nw]; q++) total = total + vals[q];
             __syncthreads();
         }
-        if (bdy == 0) {
+        if (bi == 0) {
             // calculatereactionforceFast.m:3-5 over body 0's family: F_z * cellVolume (fixed tree)
             double part = 0;
-            for (int t = threadIdx.x; t < nf; t += blockDim.x) part = part + P.F2[S.fam[0][t]] * P.V;
+            for (int t = threadIdx.x; t < nf; t += blockDim.x) part = part + P.F2[B.fam[t]] * P.V;
             rs = bbpd::block_sum_fixed<1024>(part, vals);
         }
     }
     if (threadIdx.x == 0) {
-        if (bdy < S.nBodies) P.ds->reaction[bdy] = total;
-        if (bdy == 0) P.ds->reaction_static = rs;
+        if (bi >= 0) RS.reaction[bi] = total;
+        if (bi == 0) RS.reaction_static = rs;
         __threadfence();
-        const unsigned ticket = atomicAdd(&P.ds->done, 1u);
-        last_block = (ticket == gridDim.x - 1);
+        const unsigned need = (unsigned)(RG.nBodies > 0 ? RG.nBodies : 1);
+        const unsigned ticket = atomicAdd(&RS.done, 1u);
+        last_block = (ticket == need - 1);
     }
     __syncthreads();
     if (last_block && threadIdx.x == 0) {
         __threadfence();
-        volatile DevScalars *vds = P.ds;
+        volatile RegionState *vrs = &RS;
         auto comp = [&](int slot, int d) -> double {
             if (MIXED) {
                 const volatile double *pp = reinterpret_cast<const volatile double *>(pos + slot);
@@ -800,30 +853,56 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P, StepEnd S)
             const volatile double *pu = uu[d];
             return pu[slot];
         };
-        const double defl = S.ref_slot >= 0 ? comp(S.ref_slot, 2) : 0.0;
-        const double cmod = (S.cmod1 >= 0 && S.cmod2 >= 0) ? comp(S.cmod2, 0) - comp(S.cmod1, 0) : 0.0;
-        double *h = S.hist + (size_t)(it - 1) % HIST_CAP * N_HIST;
-        h[0] = S.nBodies > 0 ? vds->reaction[0] : 0.0;
-        h[1] = vds->reaction_static;
+        const double defl = RG.ref_slot >= 0 ? comp(RG.ref_slot, 2) : 0.0;
+        const double cmod = (RG.cmod1 >= 0 && RG.cmod2 >= 0) ? comp(RG.cmod2, 0) - comp(RG.cmod1, 0) : 0.0;
+        const double r0 = RG.nBodies > 0 ? vrs->reaction[0] : 0.0;
+        const double rstat = vrs->reaction_static;
+        double *h = P.hist + (size_t)(RG.hist0 + (it - 1) % RG.hist_cap) * N_HIST;
+        h[0] = r0;
+        h[1] = rstat;
         h[2] = defl;
         h[3] = cmod;
-        vds->done = 0u;
-        vds->step = it;
+        // stop criterion, judged on converged states: the ends of load increments
+        if (RG.schedule == 1 && it % RG.ipi == 0 && !vrs->finished) {
+            const double load = fabs(rstat);
+            const double peak = vrs->peak;
+            bool stop = false;
+            if (RG.stop_frac > 0 && peak > 0 && load < RG.stop_frac * peak) stop = true;
+            if (RG.stop_disp > 0 && fabs(schedule_disp(RG, it)) >= RG.stop_disp) stop = true;
+            if (load > peak) vrs->peak = load;
+            if (stop) vrs->finished = 1;
+        }
+        vrs->done = 0u;
+        vrs->step = it;
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned ticket = atomicAdd(P.tick_done, 1u);
+        last_all = (ticket == (unsigned)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last_all && threadIdx.x == 0) {
+        __threadfence();
+        *P.tick_done = 0u;
+        *P.tick = *P.tick + 1;
     }
 }
 
 // observables over entries: softening flags, broken (H == 1 or failed), failed; each bond twice
 template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_count(Params P, unsigned long long *out)
+__global__ __launch_bounds__(BLOCK) void k_count(Params P, int ri, unsigned long long *out)
 {
-    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    const Region &RG = P.reg[ri];
+    const int n = (RG.blk0 + blockIdx.x) * BLOCK + threadIdx.x;
+    const bool real = n - RG.base < RG.nN;
     unsigned long long cs = 0, cb = 0, cf = 0;
-    if (n < P.nSlots && P.noff > 0) {
+    if (real && P.stencil) {
         // stencil layout: history per (slot, offset), sign bit = flag (failed for the brittle law)
         const uint4 msk = P.smask[n];
         const uint32_t mw[4] = {msk.x, msk.y, msk.z, msk.w};
-        const long long ebase = (long long)(n >> 6) * P.noff * SLICE + (n & (SLICE - 1));
-        for (int o = 0; o < P.noff; o++) {
+        const int noff = RG.noff;
+        const long long ebase = RG.e0 + (long long)((n - RG.base) >> 6) * noff * SLICE + (n & (SLICE - 1));
+        for (int o = 0; o < noff; o++) {
             if (!((mw[o >> 5] >> (o & 31)) & 1u)) continue;
             const long long e = ebase + (long long)o * SLICE;
             double h;
@@ -835,7 +914,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(Params P, unsigned long long *o
             cf += failed ? 1 : 0;
             cb += (failed || std::fabs(h) == 1.0) ? 1 : 0;
         }
-    } else if (n < P.nSlots) {
+    } else if (real) {
         const int len = P.rowlen[n];
         long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
         for (int kk = 0; kk < len; kk++, e += SLICE) {
@@ -916,13 +995,24 @@ struct bbpd_solver {
     std::vector<long long> entry_i, entry_j; // bond -> entry index at its two ends
     std::vector<void *> owned;
     Params P{};
-    StepEnd S{};
     size_t lds_bytes = 0;
-    DevScalars *ds = nullptr;
     unsigned long long *counts = nullptr;
-    double *hist = nullptr;
-    long long hist_read = 0; // steps already returned by bbpd_solver_history
     size_t device_bytes = 0;
+    // regions (a single problem is region 0)
+    std::vector<Region> reg;            // host copies of the descriptors
+    std::vector<char> active;           // region in use
+    std::vector<long long> hist_read;   // per region: steps already returned by the history call
+    Region *d_reg = nullptr;
+    RegionState *d_rst = nullptr;
+    int *d_blkreg = nullptr, *d_grid = nullptr, *d_active = nullptr;
+    int2 *d_se = nullptr;
+    Body *d_bodies = nullptr;
+    long long *d_tick = nullptr;
+    unsigned *d_tick_done = nullptr;
+    double *d_hist = nullptr;
+    int grid_n = 0;   // launch blocks of the per-slot kernels (a multiple of 8)
+    int nse = 0;      // step-end blocks
+    int nactive = 0;  // active regions
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -975,7 +1065,7 @@ KernelFn pick_bond_kernel(const bbpd_solver *s)
 
 int launch_step(bbpd_solver *s, bool allow_timing)
 {
-    const unsigned grid = (unsigned)(s->nSlots / BLOCK);
+    if (s->nactive == 0) return BBPD_OK; // nothing to step
     KernelFn kb = pick_bond_kernel(s);
     const bool timed = allow_timing && s->timing;
     if (timed) {
@@ -992,7 +1082,7 @@ int launch_step(bbpd_solver *s, bool allow_timing)
         }
         BBPD_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     }
-    hipLaunchKernelGGL(kb, dim3(grid), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
+    hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
     BBPD_HIP(hipGetLastError());
     if (timed) {
         BBPD_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
@@ -1000,16 +1090,14 @@ int launch_step(bbpd_solver *s, bool allow_timing)
     }
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     if (s->integrator == BBPD_INTEGRATOR_ADR) {
-        hipLaunchKernelGGL(k_adr_cn, dim3(1), dim3(BLOCK), 0, s->stream, s->P, (int)grid);
+        hipLaunchKernelGGL(k_adr_cn, dim3(s->nactive), dim3(BLOCK), 0, s->stream, s->P);
         BBPD_HIP(hipGetLastError());
-        const unsigned g2 = bbpd::grid_for(s->nN, BLOCK);
-        if (mixed) hipLaunchKernelGGL(k_adr_update<true>, dim3(g2), dim3(BLOCK), 0, s->stream, s->P);
-        else hipLaunchKernelGGL(k_adr_update<false>, dim3(g2), dim3(BLOCK), 0, s->stream, s->P);
+        if (mixed) hipLaunchKernelGGL(k_adr_update<true>, dim3(s->grid_n), dim3(BLOCK), 0, s->stream, s->P);
+        else hipLaunchKernelGGL(k_adr_update<false>, dim3(s->grid_n), dim3(BLOCK), 0, s->stream, s->P);
         BBPD_HIP(hipGetLastError());
     }
-    const unsigned nb = (unsigned)std::max(s->S.nBodies, 1);
-    if (mixed) hipLaunchKernelGGL(k_step_end<true>, dim3(nb), dim3(1024), 0, s->stream, s->P, s->S);
-    else hipLaunchKernelGGL(k_step_end<false>, dim3(nb), dim3(1024), 0, s->stream, s->P, s->S);
+    if (mixed) hipLaunchKernelGGL(k_step_end<true>, dim3(s->nse), dim3(1024), 0, s->stream, s->P);
+    else hipLaunchKernelGGL(k_step_end<false>, dim3(s->nse), dim3(1024), 0, s->stream, s->P);
     BBPD_HIP(hipGetLastError());
     return BBPD_OK;
 }
@@ -1508,7 +1596,6 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     // ---- device allocations
     BBPD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     Params &P = s->P;
-    P.nN = nN;
     P.nSlots = nSlots;
     int rc = 0;
     double *Xd, *ud = nullptr, *vd, *Fd, *Fpd = nullptr, *rhod, *massd = nullptr;
@@ -1555,9 +1642,9 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     if (sp.ok && mixed && (rc = dalloc(&tabfd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
     double *partd;
     if ((rc = dalloc(&partd, 2 * (size_t)(nSlots / BLOCK) + 2, s->owned))) return fail(rc);
-    if ((rc = dalloc(&s->ds, 1, s->owned))) return fail(rc);
+
     if ((rc = dalloc(&s->counts, 4, s->owned))) return fail(rc);
-    if ((rc = dalloc(&s->hist, (size_t)HIST_CAP * N_HIST, s->owned))) return fail(rc);
+    if ((rc = dalloc(&s->d_hist, (size_t)HIST_CAP * N_HIST, s->owned))) return fail(rc);
 
     // ---- host staging in slot order
     {
@@ -1666,9 +1753,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         t.insert(t.end(), tabp.begin(), tabp.end());
         if (!t.empty()) BBPD_HIP(hipMemcpy(tabd, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
     }
-    DevScalars zs{};
-    BBPD_HIP(hipMemcpy(s->ds, &zs, sizeof(zs), hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemset(s->hist, 0, sizeof(double) * (size_t)HIST_CAP * N_HIST));
+    BBPD_HIP(hipMemset(s->d_hist, 0, sizeof(double) * (size_t)HIST_CAP * N_HIST));
 
     auto comp = [&](double *base, int d) { return base ? base + (size_t)nSlots * d : nullptr; };
     P.X0 = comp(Xd, 0);
@@ -1714,9 +1799,6 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.hmr = hmr;
     P.hpr = hpr;
     P.two_r = two_r;
-    P.s0 = pr->s0 ? 0.0 : pr->s0_scalar;
-    P.sc = pr->sc ? 0.0 : pr->sc_scalar;
-    P.s1 = pr->s1;
     P.k = pr->k;
     P.alpha = pr->alpha;
     P.denom_exp = 1 - std::exp(-pr->k);
@@ -1725,25 +1807,52 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.DT = pr->DT;
     P.D = pr->DAMPING;
     P.partials = partd;
-    P.ds = s->ds;
-    P.noff = sp.ok ? sp.noff : 0;
-    P.kg = sp.G;
+    P.stencil = sp.ok ? 1 : 0;
     P.soff = stoffd;
     P.sxi = sxid;
     P.svcf = svcfd;
     P.smask = maskd;
     P.hs = hsd;
     P.tabf = tabfd;
-    P.ts0 = (int)tabc.size();
-    P.tsc = (int)(tabc.size() + (sp.ok ? sp.ts0.size() : 0));
     s->lds_bytes = sp.ok ? (mixed ? sizeof(float) : sizeof(double)) * ntab : (mixed ? 0 : sizeof(double) * ntab);
 
-    // ---- contact bodies and observables
-    StepEnd &S = s->S;
-    S.nBodies = pr->nBodies;
+    // ---- region 0: contact bodies, load schedule, observables
+    Region R0{};
+    R0.base = 0;
+    R0.nN = nN;
+    R0.nSlots = nSlots;
+    R0.blk0 = 0;
+    R0.nblk = nSlots / BLOCK;
+    R0.e0 = 0;
+    R0.noff = sp.ok ? sp.noff : 0;
+    R0.off0 = 0;
+    R0.kg = sp.G;
+    R0.tab0 = 0;
+    R0.ntab = (int)(tabc.size() + tabp.size());
+    R0.ts0 = (int)tabc.size();
+    R0.tsc = (int)(tabc.size() + (sp.ok ? sp.ts0.size() : 0));
+    R0.s0 = pr->s0 ? 0.0 : pr->s0_scalar;
+    R0.sc = pr->sc ? 0.0 : pr->sc_scalar;
+    R0.s1 = pr->s1;
+    R0.body0 = 0;
+    R0.nBodies = pr->nBodies;
+    R0.schedule = pr->schedule;
+    R0.nTimeSteps = pr->nTimeSteps;
+    R0.ipi = pr->itersPerIncrement > 0 ? pr->itersPerIncrement : 1;
+    R0.applied = pr->appliedDisplacement;
+    R0.increment = pr->increment;
+    auto slot_or_none = [&](int id1) { return (id1 >= 1 && id1 <= nN) ? s->slot_of[id1 - 1] : -1; };
+    R0.ref_slot = slot_or_none(pr->referenceNode);
+    R0.cmod1 = slot_or_none(pr->cmodNode1);
+    R0.cmod2 = slot_or_none(pr->cmodNode2);
+    R0.hist0 = 0;
+    R0.hist_cap = HIST_CAP;
+    R0.stop_frac = 0;
+    R0.stop_disp = 0;
     // the bodies are projected concurrently (one block each), so a node may belong to one family
     // only; the reference applies them in sequence, which is the same for disjoint families
     std::vector<unsigned char> in_family(nN, 0);
+    std::vector<Body> hb(std::max(pr->nBodies, 1));
     for (int bdy = 0; bdy < pr->nBodies; bdy++) {
         int nf = pr->nFamily[bdy];
         BBPD_REQUIRE(nf >= 0 && (nf == 0 || pr->family[bdy]), "contact family missing");
@@ -1766,12 +1875,15 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         int *fd;
         if ((rc = dalloc(&fd, fs.size(), s->owned))) return fail(rc);
         BBPD_HIP(hipMemcpy(fd, fs.data(), sizeof(int) * fs.size(), hipMemcpyHostToDevice));
-        S.fam[bdy] = fd;
-        S.nFam[bdy] = nf;
-        S.cx[bdy] = pr->centreX[bdy];
-        S.cz[bdy] = pr->centreZ[bdy];
-        S.R[bdy] = pr->radius[bdy];
-        S.moves[bdy] = pr->moves[bdy];
+        Body &B = hb[bdy];
+        B.fam = fd;
+        B.nFam = nf;
+        B.cx = pr->centreX[bdy];
+        B.cz = pr->centreZ[bdy];
+        B.R = pr->radius[bdy];
+        B.moves = pr->moves[bdy];
+        B.region = 0;
+        B.index = bdy;
     }
     if (pr->integrator == BBPD_INTEGRATOR_ADR && pr->nBodies > 0) {
         std::vector<unsigned char> hc(nSlots);
@@ -1780,20 +1892,58 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
             if (in_family[n]) hc[s->slot_of[n]] |= CONTACT_BIT;
         BBPD_HIP(hipMemcpy(consd, hc.data(), nSlots, hipMemcpyHostToDevice));
     }
-    S.schedule = pr->schedule;
-    S.nTimeSteps = pr->nTimeSteps;
-    S.ipi = pr->itersPerIncrement > 0 ? pr->itersPerIncrement : 1;
-    S.applied = pr->appliedDisplacement;
-    S.increment = pr->increment;
-    auto slot_or_none = [&](int id1) { return (id1 >= 1 && id1 <= nN) ? s->slot_of[id1 - 1] : -1; };
-    S.ref_slot = slot_or_none(pr->referenceNode);
-    S.cmod1 = slot_or_none(pr->cmodNode1);
-    S.cmod2 = slot_or_none(pr->cmodNode2);
-    S.hist = s->hist;
+    {
+        // descriptors, state, block tables, step-end jobs, parity tick
+        const int nblk = nSlots / BLOCK;
+        std::vector<int> blkreg(nblk, 0), grid(nblk);
+        std::iota(grid.begin(), grid.end(), 0);
+        std::vector<int2> se;
+        for (int bdy = 0; bdy < std::max(pr->nBodies, 1); bdy++) se.push_back(make_int2(0, pr->nBodies > 0 ? bdy : -1));
+        int act = 0;
+        if ((rc = dalloc(&s->d_reg, 1, s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_rst, 1, s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_blkreg, nblk, s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_grid, nblk, s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_active, 1, s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_se, se.size(), s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_bodies, hb.size(), s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_tick, 1, s->owned))) return fail(rc);
+        if ((rc = dalloc(&s->d_tick_done, 1, s->owned))) return fail(rc);
+        RegionState z{};
+        long long t0 = 0;
+        unsigned u0 = 0;
+        BBPD_HIP(hipMemcpy(s->d_reg, &R0, sizeof(Region), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_rst, &z, sizeof(RegionState), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_blkreg, blkreg.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_grid, grid.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_active, &act, sizeof(int), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int2) * se.size(), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_bodies, hb.data(), sizeof(Body) * hb.size(), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_tick, &t0, sizeof(t0), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_tick_done, &u0, sizeof(u0), hipMemcpyHostToDevice));
+        s->reg.assign(1, R0);
+        s->active.assign(1, 1);
+        s->hist_read.assign(1, 0);
+        s->grid_n = nblk;
+        s->nse = (int)se.size();
+        s->nactive = 1;
+        P.reg = s->d_reg;
+        P.rst = s->d_rst;
+        P.blkreg = s->d_blkreg;
+        P.grid = s->d_grid;
+        P.active = s->d_active;
+        P.se = s->d_se;
+        P.nse = s->nse;
+        P.bodies = s->d_bodies;
+        P.tick = s->d_tick;
+        P.tick_done = s->d_tick_done;
+        P.hist = s->d_hist;
+    }
 
     s->device_bytes = sizeof(NodeRec) * (size_t)nSlots * (lattice ? 2 : 3) +
                       sizeof(double) * (size_t)nSlots * (3 * (mixed ? 3 : 4) + 2 + (Fpd ? 3 : 0)) + nSlots +
-                      sizeof(uint4) * (size_t)nE + (cstrd ? 8 * nE : 0) + (s0d ? ew * nE : 0) + (scd ? ew * nE : 0);
+                      (sp.ok ? (mixed ? 4 : 8) * (size_t)nE + sizeof(uint4) * (size_t)nSlots : sizeof(uint4) * (size_t)nE) +
+                      (cstrd ? 8 * nE : 0) + (s0d ? ew * nE : 0) + (scd ? ew * nE : 0);
 
     s->ev.resize(2048);
     for (auto &e : s->ev) BBPD_HIP(hipEventCreate(&e));
@@ -1868,24 +2018,26 @@ extern "C" int bbpd_solver_sync(bbpd_solver *s)
 
 extern "C" void *bbpd_solver_stream(bbpd_solver *s) { return s ? (void *)s->stream : nullptr; }
 
-extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
+namespace {
+
+int region_observables(bbpd_solver *s, int ri, bbpd_observables *o)
 {
-    BBPD_REQUIRE(s && o, "null argument");
     BBPD_HIP(hipSetDevice(s->device));
+    const Region &RG = s->reg[ri];
     BBPD_HIP(hipMemsetAsync(s->counts, 0, sizeof(unsigned long long) * 4, s->stream));
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
-    unsigned g = (unsigned)(s->nSlots / BLOCK);
-    if (mixed) hipLaunchKernelGGL(k_count<true>, dim3(g), dim3(BLOCK), 0, s->stream, s->P, s->counts);
-    else hipLaunchKernelGGL(k_count<false>, dim3(g), dim3(BLOCK), 0, s->stream, s->P, s->counts);
+    const unsigned g = (unsigned)RG.nblk;
+    if (mixed) hipLaunchKernelGGL(k_count<true>, dim3(g), dim3(BLOCK), 0, s->stream, s->P, ri, s->counts);
+    else hipLaunchKernelGGL(k_count<false>, dim3(g), dim3(BLOCK), 0, s->stream, s->P, ri, s->counts);
     BBPD_HIP(hipGetLastError());
     unsigned long long c[4];
-    DevScalars d;
+    RegionState d;
     BBPD_HIP(hipMemcpyAsync(c, s->counts, sizeof(c), hipMemcpyDeviceToHost, s->stream));
-    BBPD_HIP(hipMemcpyAsync(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost, s->stream));
+    BBPD_HIP(hipMemcpyAsync(&d, s->d_rst + ri, sizeof(d), hipMemcpyDeviceToHost, s->stream));
     BBPD_HIP(hipStreamSynchronize(s->stream));
     std::memset(o, 0, sizeof(*o));
     o->step = d.step;
-    o->reaction = s->S.nBodies > 0 ? d.reaction[0] : 0.0;
+    o->reaction = RG.nBodies > 0 ? d.reaction[0] : 0.0;
     o->reaction_static = d.reaction_static;
     o->cn = d.cn;
     o->n_softening = (int64_t)(c[0] / 2);
@@ -1893,11 +2045,20 @@ extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
     o->n_failed = (int64_t)(c[2] / 2);
     if (d.step > 0) {
         double h[N_HIST];
-        BBPD_HIP(hipMemcpy(h, s->hist + (size_t)((d.step - 1) % HIST_CAP) * N_HIST, sizeof(h), hipMemcpyDeviceToHost));
+        BBPD_HIP(hipMemcpy(h, s->d_hist + (size_t)(RG.hist0 + (d.step - 1) % RG.hist_cap) * N_HIST, sizeof(h),
+                           hipMemcpyDeviceToHost));
         o->deflection = h[2];
         o->cmod = h[3];
     }
     return BBPD_OK;
+}
+
+} // namespace
+
+extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
+{
+    BBPD_REQUIRE(s && o, "null argument");
+    return region_observables(s, 0, o);
 }
 
 namespace {
@@ -1984,16 +2145,30 @@ int put_pos(bbpd_solver *s, int buf_id, const double *src)
     return BBPD_OK;
 }
 
-long long current_step(bbpd_solver *s, int *rc)
+// steps completed by a region
+long long current_step(bbpd_solver *s, int *rc, int ri = 0)
 {
-    DevScalars d;
-    hipError_t e = hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost);
+    RegionState d;
+    hipError_t e = hipMemcpy(&d, s->d_rst + ri, sizeof(d), hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
         *rc = bbpd::hip_fail(e, "hipMemcpy (step counter)");
         return 0;
     }
     *rc = BBPD_OK;
     return d.step;
+}
+
+// the position buffer the next step reads (launch parity)
+int current_buffer(bbpd_solver *s, int *rc)
+{
+    long long t = 0;
+    hipError_t e = hipMemcpy(&t, s->d_tick, sizeof(t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        *rc = bbpd::hip_fail(e, "hipMemcpy (launch counter)");
+        return 0;
+    }
+    *rc = BBPD_OK;
+    return (int)(t & 1);
 }
 
 } // namespace
@@ -2004,9 +2179,8 @@ extern "C" int bbpd_solver_get_nodal(bbpd_solver *s, double *u, double *v, doubl
     BBPD_HIP(hipSetDevice(s->device));
     BBPD_HIP(hipStreamSynchronize(s->stream));
     int rc = 0;
-    const long long step = current_step(s, &rc);
+    const int cur = current_buffer(s, &rc);
     if (rc) return rc;
-    const int cur = (int)(step & 1);
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
     double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
@@ -2109,12 +2283,13 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
     BBPD_HIP(hipSetDevice(s->device));
     BBPD_HIP(hipStreamSynchronize(s->stream));
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
-    const int cur = (int)(step & 1);
+    int rc = 0;
+    const int cur = current_buffer(s, &rc);
+    if (rc) return rc;
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
     double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
     double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
     double *fpc[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
-    int rc;
     if (u && (rc = mixed ? put_pos(s, cur, u) : put_soa(s, uc, u))) return rc;
     if (!mixed && x && (rc = put_pos(s, cur, x))) return rc;
     if (v && (rc = put_soa(s, vc, v))) return rc;
@@ -2167,38 +2342,50 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
         }
         BBPD_HIP(hipMemcpy(s->P.rec, r.data(), sizeof(uint4) * nE, hipMemcpyHostToDevice));
     }
-    DevScalars d;
-    BBPD_HIP(hipMemcpy(&d, s->ds, sizeof(d), hipMemcpyDeviceToHost));
+    RegionState d;
+    BBPD_HIP(hipMemcpy(&d, s->d_rst, sizeof(d), hipMemcpyDeviceToHost));
     d.step = step;
-    BBPD_HIP(hipMemcpy(s->ds, &d, sizeof(d), hipMemcpyHostToDevice));
-    s->hist_read = step;
+    d.done = 0u;
+    BBPD_HIP(hipMemcpy(s->d_rst, &d, sizeof(d), hipMemcpyHostToDevice));
+    s->hist_read[0] = step;
     return BBPD_OK;
 }
 
-extern "C" int bbpd_solver_history(bbpd_solver *s, double *out_rows, int cap, int *n_out)
+namespace {
+
+int region_history(bbpd_solver *s, int ri, double *out_rows, int cap, int *n_out)
 {
-    BBPD_REQUIRE(s && n_out && cap >= 0, "bad arguments");
     BBPD_HIP(hipSetDevice(s->device));
     BBPD_HIP(hipStreamSynchronize(s->stream));
     int rc = 0;
-    const long long step = current_step(s, &rc);
+    const long long step = current_step(s, &rc, ri);
     if (rc) return rc;
-    long long from = std::max(s->hist_read, step - (long long)HIST_CAP);
+    const Region &RG = s->reg[ri];
+    const long long hc = RG.hist_cap;
+    long long from = std::max(s->hist_read[ri], step - hc);
     long long cnt = std::min((long long)cap, step - from);
     // copy only the ring rows [from, from + cnt), in at most two pieces
     std::vector<double> rows((size_t)std::max(cnt, 0LL) * N_HIST);
     long long done = 0;
     while (done < cnt) {
-        const long long r0 = (from + done) % HIST_CAP;
-        const long long piece = std::min(cnt - done, (long long)HIST_CAP - r0);
-        BBPD_HIP(hipMemcpy(rows.data() + (size_t)done * N_HIST, s->hist + (size_t)r0 * N_HIST,
+        const long long r0 = (from + done) % hc;
+        const long long piece = std::min(cnt - done, hc - r0);
+        BBPD_HIP(hipMemcpy(rows.data() + (size_t)done * N_HIST, s->d_hist + (size_t)(RG.hist0 + r0) * N_HIST,
                            sizeof(double) * N_HIST * (size_t)piece, hipMemcpyDeviceToHost));
         done += piece;
     }
     if (out_rows && cnt > 0) std::memcpy(out_rows, rows.data(), sizeof(double) * rows.size());
-    s->hist_read = from + cnt;
+    s->hist_read[ri] = from + cnt;
     *n_out = (int)cnt;
     return BBPD_OK;
+}
+
+} // namespace
+
+extern "C" int bbpd_solver_history(bbpd_solver *s, double *out_rows, int cap, int *n_out)
+{
+    BBPD_REQUIRE(s && n_out && cap >= 0, "bad arguments");
+    return region_history(s, 0, out_rows, cap, n_out);
 }
 
 extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_bytes, int64_t *n_entries,
