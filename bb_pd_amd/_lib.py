@@ -56,6 +56,22 @@ class Observables(C.Structure):
     ]
 
 
+class BatchSpec(C.Structure):
+    _fields_ = [
+        ("maxRegions", C.c_int), ("maxSlots", C.c_int64), ("historyRows", C.c_int), ("DX", C.c_double),
+        ("shape", C.POINTER(Problem)),
+    ]
+
+
+class RegionStatus(C.Structure):
+    _fields_ = [
+        ("active", C.c_int), ("finished", C.c_int), ("step", C.c_int64), ("peak", C.c_double),
+        ("reaction_static", C.c_double), ("nNodes", C.c_int64), ("nBonds", C.c_int64),
+    ]
+
+
+ERR_FULL = -5
+
 # name -> (restype, argtypes); every symbol include/bbpd.h declares
 _V, _I, _D, _L, _P = C.c_void_p, C.c_int, C.c_double, C.c_int64, C.c_void_p
 SIGNATURES = {
@@ -89,6 +105,14 @@ SIGNATURES = {
     "bbpd_solver_history": (_I, [_V, _P, _I, C.POINTER(_I)]),
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_layout": (C.c_char_p, [_V]),
+    "bbpd_batch_create": (_I, [_P, _I, _I, C.POINTER(_V)]),
+    "bbpd_batch_admit": (_I, [_V, _P, _D, _D, C.POINTER(_I)]),
+    "bbpd_batch_retire": (_I, [_V, _I]),
+    "bbpd_batch_status": (_I, [_V, _P, _I]),
+    "bbpd_batch_observables": (_I, [_V, _I, _P]),
+    "bbpd_batch_history": (_I, [_V, _I, _P, _I, C.POINTER(_I)]),
+    "bbpd_batch_get_nodal": (_I, [_V, _I, _P, _P, _P, _P]),
+    "bbpd_batch_get_bonds": (_I, [_V, _I, _P, _P, _P]),
     "bbpd_solver_set_timing": (_I, [_V, _I]),
     "bbpd_solver_kernel_time": (_I, [_V, _dp, C.POINTER(_L)]),
 }

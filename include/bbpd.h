@@ -249,6 +249,54 @@ const char *bbpd_solver_layout(bbpd_solver *s);
 int bbpd_solver_set_timing(bbpd_solver *s, int on);
 int bbpd_solver_kernel_time(bbpd_solver *s, double *avg_ms, int64_t *launches);
 
+/* ============================================================================================
+ * Batches: many problems stepped together in one launch of each kernel.  No counterpart in the
+ * reference, whose only multi-specimen workflow runs specimens one after another
+ * (solvers/scripts/randomfield/dynamicsolverdisplacementcontrolledRF.m:20-114, its CMOD stop being
+ * the analogue of the stop criterion here).  A batch is a pool of regions: problems are admitted
+ * into free node slots while others run, step together, and are retired to free their slots.
+ * Batches use the stencil layout, so every problem must be a full lattice box with a BONDLIST sorted
+ * by node id (what the input generator produces).  bbpd_solver_step / run_graph / sync / info /
+ * set_timing / kernel_time / destroy work on a batch as on a single problem.
+ * ========================================================================================== */
+#define BBPD_ERR_FULL (-5) /* the batch has no free region or slots for the problem now */
+
+typedef struct bbpd_batch_spec {
+    int maxRegions;            /* problems held at once */
+    int64_t maxSlots;          /* node slots over all problems (a problem takes nNodes rounded up to 256) */
+    int historyRows;           /* per-problem history ring rows (0: 65536) */
+    double DX;                 /* lattice spacing every problem shares */
+    const bbpd_problem *shape; /* every admitted problem shares its cellVolume, horizon, RADIJ, law, k,
+                                  alpha, sc_steel, failureFunctionality, integrator, DT, DAMPING, and
+                                  whether s0 / sc are per bond (arrays) or scalars */
+} bbpd_batch_spec;
+
+typedef struct bbpd_region_status {
+    int active;             /* the region holds a problem */
+    int finished;           /* its stop criterion was met */
+    int64_t step;           /* steps done */
+    double peak;            /* largest static reaction at a load-increment end */
+    double reaction_static; /* static reaction of the last step */
+    int64_t nNodes, nBonds;
+} bbpd_region_status;
+
+int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int device, bbpd_solver **out);
+/* Admit a problem: it starts at step 0 with the next launch.  stopFraction / stopDisplacement (0 =
+ * none): the problem is marked finished at the first load-increment end where its static reaction
+ * is below stopFraction of its peak, or the load point has travelled stopDisplacement.  Returns
+ * BBPD_ERR_FULL when no region or slot range is free now. */
+int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *problem, double stopFraction, double stopDisplacement,
+                     int *region);
+int bbpd_batch_retire(bbpd_solver *s, int region);
+/* status of regions 0 .. n-1 (n <= maxRegions) in one device read */
+int bbpd_batch_status(bbpd_solver *s, bbpd_region_status *status, int n);
+int bbpd_batch_observables(bbpd_solver *s, int region, bbpd_observables *obs);
+int bbpd_batch_history(bbpd_solver *s, int region, double *rows, int cap, int *n_out);
+int bbpd_batch_get_nodal(bbpd_solver *s, int region, double *nodalDisplacement, double *nodalVelocity,
+                         double *nodalForce, double *deformedCoordinates);
+int bbpd_batch_get_bonds(bbpd_solver *s, int region, double *bondSofteningFactor, double *flagBondSoftening,
+                         double *fail);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,9 +41,10 @@ def test_host_only_calls_work_without_gpu():
 
 
 def test_struct_layouts_match_the_header(tmp_path):
-    """ctypes mirrors of bbpd_problem / bbpd_beam_spec / bbpd_observables have the C layout."""
+    """ctypes mirrors of the header's structs have the C layout."""
     import subprocess
-    structs = {"bbpd_problem": _lib.Problem, "bbpd_beam_spec": _lib.BeamSpec, "bbpd_observables": _lib.Observables}
+    structs = {"bbpd_problem": _lib.Problem, "bbpd_beam_spec": _lib.BeamSpec, "bbpd_observables": _lib.Observables,
+               "bbpd_batch_spec": _lib.BatchSpec, "bbpd_region_status": _lib.RegionStatus}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "bbpd.h"', "int main(void){"]
     for cname, py in structs.items():
         src.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
