@@ -257,6 +257,19 @@ def case_exponential_dx1p25(**kw) -> BeamCase:
     return build_beam("beam500x100x50_dx1.25_offset_notch_exponential", nx, 40, 80, 1.25e-3, **args)
 
 
+def case_size_effect(depth_mm: int, seed: int = 0, **kw) -> BeamCase:
+    """Geometrically similar beams of the size-effect family: depth 50 / 100 / 200 / 400 mm, length
+    5 x depth, width 50 mm, dx = 5 mm (5k / 20k / 80k / 320k nodes), unnotched, bilinear law with the
+    seeded +-1% per-bond s0 perturbation (s0 = 1e-4, sc = 2e-3), ADR; supports at 5% of the length
+    from each end as for the other beams."""
+    DX = 5e-3
+    nz = int(round(depth_mm * 1e-3 / DX))
+    nx, ny = 5 * nz, 10
+    args = dict(law=_lib.LAW_BILINEAR, s0=1.0e-4, s0_perturb=0.01, seed=seed, sc=2.0e-3)
+    args.update(kw)
+    return build_beam(f"beam_d{depth_mm}_seed{seed}", nx, ny, nz, DX, **args)
+
+
 def to_problem(case: BeamCase, keep: list) -> _lib.Problem:
     """Fill the C problem struct; `keep` receives every array the struct points into."""
     def p(a):
