@@ -1,0 +1,125 @@
+"""Continuous-batching service for fracture solves on one GPU.
+
+Requests arrive over time (Poisson, or all at once for a closed-loop capacity run).  Each is a
+three-point-bend beam solve that runs until its load falls below a fraction of its peak after the
+peak, or its load point has travelled a set distance.  The service keeps one resident batch
+(`Batch`): an arriving request is admitted as soon as a region and enough node slots are free
+(admission by device-memory footprint: the batch's slot capacity is sized from an HBM budget);
+all admitted requests step together in one launch per kernel, each at its own step count; finished
+ones are retired and their slots reused.  Nothing of a request's state leaves the device between
+its admission and its retirement.
+
+The reference has no service; its only multi-specimen workflow runs specimens one after another
+(solvers/scripts/randomfield/dynamicsolverdisplacementcontrolledRF.m:20-114), stopping each on a
+CMOD criterion, the analogue of the stop criterion here.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, beam
+from .batch import Batch, BatchFull
+
+
+@dataclass
+class Request:
+    rid: int
+    case: beam.BeamCase
+    arrival: float = 0.0         # s after the start of the run
+    admitted: float = math.nan
+    finished: float = math.nan
+    region: int = -1
+    steps: int = 0
+    peak: float = 0.0
+    history: dict = field(default_factory=dict)
+
+    @property
+    def time_to_solution(self) -> float:
+        return self.finished - self.arrival
+
+
+def request_case(rng: np.random.Generator, ipi: int, increment_um: float) -> beam.BeamCase:
+    """One request of the serving workload: depth in {50, 100, 200, 400} mm, notch ratio in
+    {0, 0.2, 0.4} (a midspan notch of that fraction of the depth), E ~ U[30, 45] GPa,
+    s0 ~ U[0.8, 1.2]e-4 (scalar), bilinear law with sc = 2e-3, dx = 5 mm, ADR on the penetrator
+    staircase of `increment_um` per `ipi` iterations."""
+    depth = int(rng.choice([50, 100, 200, 400]))
+    ratio = float(rng.choice([0.0, 0.2, 0.4]))
+    E = float(rng.uniform(30e9, 45e9))
+    s0 = float(rng.uniform(0.8e-4, 1.2e-4))
+    nz = depth // 5
+    nx = 5 * nz
+    notch = (nx / 2 + 0.5, ratio * nz + 0.5) if ratio > 0 else None
+    return beam.build_beam(f"req_d{depth}_n{ratio}_E{E / 1e9:.1f}", nx, 10, nz, 5e-3, law=_lib.LAW_BILINEAR,
+                           s0=s0, sc=2.0e-3, E=E, notch=notch, integrator=_lib.ADR, iters_per_increment=ipi,
+                           increment=-increment_um * 1e-6)
+
+
+def slot_bytes(precision: int) -> int:
+    """Device bytes a batch holds per node slot (node state, bond mask, 128 history entries)."""
+    w = 4 if precision == _lib.MIXED else 8
+    node = 2 * 32 + 8 * (3 * (3 if precision == _lib.MIXED else 4) + 2 + 3) + 1 + 16
+    return node + 128 * w
+
+
+def serve(requests: list, precision: int, hbm_budget: float, chunk: int, stop_fraction: float = 0.2,
+          stop_displacement: float = 5e-4, max_regions: int = 64, keep_history: bool = False,
+          closed_loop: bool = False, log=None) -> dict:
+    """Run the service over `requests` (their `arrival` times are honoured unless `closed_loop`,
+    in which case all are queued at t = 0).  Returns per-request times and aggregate figures."""
+    import torch
+    slots = int(hbm_budget // slot_bytes(precision))
+    pending = sorted(requests, key=lambda r: r.arrival)
+    queue: list = []
+    active: dict = {}
+    done: list = []
+    bond_steps = 0
+    with Batch(pending[0].case, precision, max_regions=max_regions, max_slots=slots,
+               history_rows=1 << 17) as b:
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while pending or queue or active:
+            t = now()
+            while pending and (closed_loop or pending[0].arrival <= t):
+                queue.append(pending.pop(0))
+            # admission, first come first served; a request that does not fit waits for slots
+            while queue:
+                r = queue[0]
+                try:
+                    r.region = b.admit(r.case, stop_fraction, stop_displacement)
+                except BatchFull:
+                    break
+                r.admitted = now()
+                active[r.region] = r
+                queue.pop(0)
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, pending[0].arrival - now()))
+                continue
+            b.run_graph(chunk, 1)
+            b.sync()
+            bond_steps += chunk * sum(r.case.nBonds for r in active.values())
+            st = b.status()
+            for reg, r in list(active.items()):
+                if st[reg]["finished"]:
+                    r.finished = now()
+                    r.steps = st[reg]["step"]
+                    r.peak = st[reg]["peak"]
+                    if keep_history:
+                        r.history = b.history(reg)
+                    b.retire(reg)
+                    del active[reg]
+                    done.append(r)
+                    if log:
+                        log(f"    request {r.rid} ({r.case.name}) done: {r.steps} steps, "
+                            f"{r.time_to_solution:.2f} s")
+        wall = now()
+        torch.cuda.synchronize()
+    tts = np.array([r.time_to_solution for r in done])
+    return dict(requests=len(done), wall_s=wall, requests_per_s=len(done) / wall,
+                bond_updates_per_s=bond_steps / wall, p50_s=float(np.percentile(tts, 50)),
+                p99_s=float(np.percentile(tts, 99)), slots=slots, done=done)
