@@ -105,6 +105,7 @@ SIGNATURES = {
     "bbpd_solver_history": (_I, [_V, _P, _I, C.POINTER(_I)]),
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_layout": (C.c_char_p, [_V]),
+    "bbpd_solver_autotune": (_I, [_V, _I, _P, C.POINTER(_I)]),
     "bbpd_batch_create": (_I, [_P, _I, _I, C.POINTER(_V)]),
     "bbpd_batch_admit": (_I, [_V, _P, _D, _D, C.POINTER(_I)]),
     "bbpd_batch_retire": (_I, [_V, _I]),

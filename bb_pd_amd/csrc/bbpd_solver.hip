@@ -308,9 +308,9 @@ __device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool
 }
 
 // the global block a launch block works on (after the XCD remap), -1 = idle
-__device__ inline int launch_block(const Params &P)
+__device__ inline int launch_block(const Params &P, bool remap = true)
 {
-    return P.grid[remap_block(blockIdx.x, gridDim.x)];
+    return P.grid[remap ? remap_block(blockIdx.x, gridDim.x) : (int)blockIdx.x];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -509,8 +509,11 @@ struct StEnt {
     NodeRec nb; // neighbour record
 };
 
-template <bool MIXED, int LAW>
-__global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
+// Launch variants, picked per solver by an autotuner: OCC is the minimum waves per SIMD the compiler
+// must allow (1 = its own choice), which bounds the registers per wave, trading instruction-level
+// for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.
+template <bool MIXED, int LAW, int OCC, bool REMAP>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) void k_bond_st(Params P)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
     __shared__ double sh[BLOCK];
@@ -520,7 +523,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_st(Params P)
     __shared__ float4 s_xi[MIXED ? ST_MAX : 1];
     __shared__ float s_vcf[MIXED ? ST_MAX : 1];
     extern __shared__ double tabs[]; // key tables (FP64, or FP32 in the mixed form)
-    const int gb = launch_block(P);
+    const int gb = launch_block(P, REMAP);
     if (gb < 0) return;
     const int n = gb * BLOCK + threadIdx.x;
     const Region &RG = P.reg[P.blkreg[gb]];
@@ -1037,6 +1040,7 @@ struct bbpd_solver {
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     int graph_version = -1;
+    int occ = 0; // occupancy variant of the stencil bond kernel (index into OCC_WAVES)
 };
 
 namespace {
@@ -1060,21 +1064,36 @@ KernelFn pick_law(int law, bool lat)
     }
 }
 
-template <bool MIXED>
-KernelFn pick_law_st(int law)
+// launch variants of the stencil bond kernel: (minimum waves per SIMD, XCD block remap)
+constexpr int N_OCC = 4;
+
+template <bool MIXED, int LAW>
+KernelFn pick_occ(int v)
 {
-    switch (law) {
-    case BBPD_LAW_LINEAR: return k_bond_st<MIXED, BBPD_LAW_LINEAR>;
-    case BBPD_LAW_BILINEAR: return k_bond_st<MIXED, BBPD_LAW_BILINEAR>;
-    case BBPD_LAW_TRILINEAR: return k_bond_st<MIXED, BBPD_LAW_TRILINEAR>;
-    default: return k_bond_st<MIXED, BBPD_LAW_EXPONENTIAL>;
+    switch (v) {
+    case 1: return k_bond_st<MIXED, LAW, 1, false>;
+    case 2: return k_bond_st<MIXED, LAW, 5, true>;
+    case 3: return k_bond_st<MIXED, LAW, 8, true>;
+    default: return k_bond_st<MIXED, LAW, 1, true>;
     }
 }
 
-KernelFn pick_bond_kernel(const bbpd_solver *s)
+template <bool MIXED>
+KernelFn pick_law_st(int law, int v)
+{
+    switch (law) {
+    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR>(v);
+    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR>(v);
+    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR>(v);
+    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL>(v);
+    }
+}
+
+KernelFn pick_bond_kernel(const bbpd_solver *s, int variant = -1)
 {
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
-    if (s->stencil) return mixed ? pick_law_st<true>(s->law) : pick_law_st<false>(s->law);
+    const int v = variant >= 0 ? variant : s->occ;
+    if (s->stencil) return mixed ? pick_law_st<true>(s->law, v) : pick_law_st<false>(s->law, v);
     return mixed ? pick_law<true>(s->law, s->lattice) : pick_law<false>(s->law, s->lattice);
 }
 
@@ -2462,6 +2481,47 @@ extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *
 extern "C" const char *bbpd_solver_layout(bbpd_solver *s)
 {
     return s ? s->layout.c_str() : "";
+}
+
+extern "C" int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *choice)
+{
+    BBPD_REQUIRE(s && reps > 0, "bad arguments");
+    if (choice) *choice = s->occ;
+    if (!s->stencil || s->nactive == 0) return BBPD_OK; // one variant only
+    BBPD_REQUIRE(s->integrator == BBPD_INTEGRATOR_ADR,
+                 "autotuning reruns the bond kernel on the current state, which only the ADR step leaves unchanged");
+    BBPD_HIP(hipSetDevice(s->device));
+    // under ADR the bond kernel writes F, the Rayleigh partials and max(H, bsf(s)) from positions it
+    // does not change: rerunning it on the same state writes the same values
+    hipEvent_t e0, e1;
+    BBPD_HIP(hipEventCreate(&e0));
+    BBPD_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int bestv = s->occ;
+    for (int v = 0; v < N_OCC; v++) {
+        KernelFn kb = pick_bond_kernel(s, v);
+        hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P); // warm-up
+        BBPD_HIP(hipGetLastError());
+        BBPD_HIP(hipEventRecord(e0, s->stream));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
+        BBPD_HIP(hipEventRecord(e1, s->stream));
+        BBPD_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        BBPD_HIP(hipEventElapsedTime(&t, e0, e1));
+        const double avg = (double)t / reps;
+        if (ms) ms[v] = avg;
+        if (avg < best) {
+            best = avg;
+            bestv = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->occ = bestv;
+    s->version++; // captured graphs hold the old kernel
+    if (choice) *choice = bestv;
+    return BBPD_OK;
 }
 
 extern "C" int bbpd_solver_set_timing(bbpd_solver *s, int on)

@@ -122,6 +122,14 @@ class Solver:
         """"stencil" or "records (<reason>)": the bond-table layout chosen at set-up."""
         return self._L.bbpd_solver_layout(self._h).decode()
 
+    def autotune(self, reps: int = 5):
+        """Time the bond-kernel variants on the current state (ADR only), keep the fastest; returns
+        (index kept, average ms per variant)."""
+        ms = (C.c_double * 4)()
+        ch = C.c_int()
+        _lib.check(self._L.bbpd_solver_autotune(self._h, reps, ms, C.byref(ch)), "bbpd_solver_autotune")
+        return ch.value, list(ms)
+
     def info(self) -> dict:
         v = [C.c_int64() for _ in range(4)]
         _lib.check(self._L.bbpd_solver_info(self._h, *[C.byref(x) for x in v]), "bbpd_solver_info")

@@ -74,6 +74,10 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=()):
     sol.sync()
     if warmup in keep:
         ckpts[warmup] = sol.checkpoint()
+    tune = None
+    if case.integrator == _lib.ADR and sol.layout() == "stencil":
+        tune = sol.autotune(5)
+        log(f"  autotune: variant {tune[0]} of {['%.3f' % x for x in tune[1]]} ms")
     sol.set_timing(True)
     dt, done = 0.0, 0
     while done < steps:
@@ -101,7 +105,8 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=()):
     achieved = model / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return dict(solver=sol, seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=case.nBonds * steps / dt,
                 kernel_ms=avg_ms, launches=launches, bytes_model=model, bytes_layout=info["bond_kernel_bytes"],
-                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, info=info, hist=hist, ckpts=ckpts)
+                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, info=info, hist=hist, ckpts=ckpts,
+                layout=sol.layout(), autotune=tune)
 
 
 def increment_curve(R: np.ndarray, ipi: int) -> np.ndarray:
@@ -309,6 +314,8 @@ def run_batch(cases, precision, steps, warmup, chunk=50):
     log(f"  admitted {len(cases)} problems in {time.time() - t0:.1f}s, info {b.info()}")
     b.step(warmup)
     b.sync()
+    tune = b.autotune(5)
+    log(f"  autotune: variant {tune[0]} of {['%.3f' % x for x in tune[1]]} ms")
     # bond-kernel duration from HIP events over eager steps (graph replays record none)
     b.set_timing(True)
     b.step(10)
@@ -330,7 +337,8 @@ def run_batch(cases, precision, steps, warmup, chunk=50):
     st = b.status()
     b.close()
     return dict(seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=nb * steps / dt, kernel_ms=avg_ms,
-                launches=launches, steps_done=[x["step"] for x in st[:len(cases)]])
+                launches=launches, steps_done=[x["step"] for x in st[:len(cases)]],
+                autotune={"variant": tune[0], "ms": tune[1]})
 
 
 def run_serve(n_requests: int, hbm_gb: float, ipi: int, increment_um: float, load: float = 0.8):
@@ -435,8 +443,9 @@ def main():
             "config": {"workload": "size-effect family: depth 50/100/200/400 mm x 16 seeds, dx 5 mm, bilinear, ADR",
                        "instances": len(cases), "nodes": sum(c.nNodes for c in cases), "bonds": nb,
                        "steps": "hipGraph-captured chunks of 50"},
-            "kernel_ms": m["kernel_ms"],
-            "fp64": {"value": f["bond_updates_per_s"], "ms_per_step": f["ms_per_step"], "kernel_ms": f["kernel_ms"]},
+            "kernel_ms": m["kernel_ms"], "autotune": m["autotune"],
+            "fp64": {"value": f["bond_updates_per_s"], "ms_per_step": f["ms_per_step"], "kernel_ms": f["kernel_ms"],
+                     "autotune": f["autotune"]},
             "mixed_over_fp64": m["bond_updates_per_s"] / f["bond_updates_per_s"],
         }), flush=True)
         return
@@ -468,7 +477,7 @@ def main():
         res = fp64 or mixed
         res["solver"].close()
         print(json.dumps({"form": args.form, "result": {k: v for k, v in res.items()
-                                                         if k not in ("solver", "info", "hist", "ckpts")}}),
+                                                         if k not in ("solver", "info", "hist", "ckpts", "autotune")}}),
               flush=True)
         return
     prec = None
@@ -519,7 +528,8 @@ def main():
         "precision": prec,
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["frac"], "traffic": traffic["bytes"] if traffic else None,
-                     "kernel": "k_bond (FP32-mixed)", "kernel_ms": head["kernel_ms"],
+                     "kernel": f"bond kernel, FP32-mixed, {head['layout']} layout", "kernel_ms": head["kernel_ms"],
+                     "autotune": {"variant": head["autotune"][0], "ms": head["autotune"][1]} if head["autotune"] else None,
                      "bytes_model": head["bytes_model"], "bytes_layout": head["bytes_layout"]},
         "cpu_baseline": cpu,
     }

@@ -244,6 +244,13 @@ int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel
  * string lives as long as the solver.  Default: stencil where it applies for the FP32-mixed form,
  * records for plain FP64; BBPD_LAYOUT=stencil or BBPD_LAYOUT=records in the environment chooses. */
 const char *bbpd_solver_layout(bbpd_solver *s);
+/* Autotuner: times each compiled launch variant of the stencil bond kernel over `reps` launches on
+ * the current state and keeps the fastest for the solver or batch.  Variants: 0 = compiler's
+ * register budget, XCD-contiguous block order; 1 = the same, blocks in launch order; 2 / 3 = at
+ * least 5 / 8 waves per SIMD (fewer registers per wave), XCD order.  ms (may be NULL) receives the 4
+ * averages, choice the index kept.  Needs the ADR integrator (whose bond kernel leaves the state
+ * unchanged when rerun); a no-op on the record layout.  No counterpart in the reference. */
+int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *choice);
 /* average duration (ms) of the bond kernel over the steps since the last call, from HIP events
  * recorded on the solver's stream around each launch (0 when timing is off) */
 int bbpd_solver_set_timing(bbpd_solver *s, int on);

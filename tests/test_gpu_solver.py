@@ -403,3 +403,25 @@ def test_layout_defaults_and_fallback(gpu):
         setattr(case, name, getattr(case, name)[order])
     with _solver_with_layout(case, _lib.MIXED, "stencil") as s:
         assert s.layout().startswith("records (rows not in ascending"), s.layout()
+
+
+def test_autotune_leaves_the_trajectory_unchanged(gpu):
+    """The autotuner reruns the bond kernel on the live ADR state; every variant computes the same
+    arithmetic, so the tuned run equals an untuned one bit for bit, in either form."""
+    case = adr_case()
+    for prec in (_lib.MIXED, _lib.FP64):
+        a = _solver_with_layout(case, prec, "stencil")
+        b = _solver_with_layout(case, prec, "stencil")
+        with a, b:
+            a.step(120)
+            choice, ms = a.autotune(3)
+            assert 0 <= choice < 4 and len(ms) == 4 and min(ms) > 0
+            a.step(130)
+            b.step(250)
+            for x, y in zip(a.nodal(), b.nodal()):
+                assert same(x, y)
+            for x, y in zip(a.bonds(), b.bonds()):
+                assert same(x, y)
+    with Solver(ec_case(_lib.LAW_BILINEAR), _lib.MIXED) as e:
+        with pytest.raises(_lib.BBPDError, match="ADR"):
+            e.autotune(3)
