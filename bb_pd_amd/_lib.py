@@ -106,6 +106,8 @@ SIGNATURES = {
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_layout": (C.c_char_p, [_V]),
     "bbpd_solver_autotune": (_I, [_V, _I, _P, C.POINTER(_I)]),
+    "bbpd_solver_damage": (_I, [_V, _P, C.POINTER(_D)]),
+    "bbpd_batch_damage": (_I, [_V, _I, _P, C.POINTER(_D)]),
     "bbpd_batch_create": (_I, [_P, _I, _I, C.POINTER(_V)]),
     "bbpd_batch_admit": (_I, [_V, _P, _D, _D, C.POINTER(_I)]),
     "bbpd_batch_retire": (_I, [_V, _I]),

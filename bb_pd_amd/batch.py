@@ -131,6 +131,14 @@ class Batch:
         _lib.check(self._L.bbpd_solver_autotune(self._h, reps, ms, C.byref(ch)), "bbpd_solver_autotune")
         return ch.value, list(ms)
 
+    def damage(self, region: int, energy: bool = True):
+        """(damage field of the region's problem, total damage energy or None)."""
+        d = np.zeros(self.cases[region].nNodes)
+        e = C.c_double()
+        _lib.check(self._L.bbpd_batch_damage(self._h, region, d.ctypes.data_as(C.c_void_p),
+                                             C.byref(e) if energy else None), "bbpd_batch_damage")
+        return d, (e.value if energy else None)
+
     def info(self) -> dict:
         v = [C.c_int64() for _ in range(4)]
         _lib.check(self._L.bbpd_solver_info(self._h, *[C.byref(x) for x in v]), "bbpd_solver_info")

@@ -942,6 +942,166 @@ __global__ __launch_bounds__(BLOCK) void k_count(Params P, int ri, unsigned long
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Damage field (calculatedamage.m:34-45) and dissipated damage energy (calculatedamageenergy.m:35-62)
+// of one region, on demand.  The reference updates a per-bond energy history every step as
+// max(de(s)); de is non-decreasing in the stretch, so the history is de at the largest stretch
+// reached.  Every softening law here is non-decreasing in the stretch on (s0, sc] and the stored
+// factor H is its largest value so far, so that stretch is the law inverted at H (bisection).
+// ------------------------------------------------------------------------------------------------
+template <int LAW>
+__device__ double law_factor(double s, double s0, double sc, double s1, double k, double alpha, double denom)
+{
+    if (LAW == BBPD_LAW_BILINEAR) return ((s - s0) / s) * (sc / (sc - s0));
+    if (LAW == BBPD_LAW_TRILINEAR) {
+        const double beta = 0.25, eta = s1 / s0;
+        if (s <= s1) return (1 - ((eta - beta) / (eta - 1) * (s0 / s))) + ((1 - beta) / (eta - 1));
+        return 1 - ((s0 * beta) / s) * ((sc - s) / (sc - s1));
+    }
+    const double t = (s - s0) / (sc - s0);
+    const double numerator = 1 - exp(-k * t);
+    const double residual = alpha * (1 - t);
+    return 1 - (s0 / s) * ((1 - (numerator / denom)) + residual) / (1 + alpha);
+}
+
+// the largest stretch a bond has reached, from its factor H (s0 when it never softened)
+template <int LAW>
+__device__ double max_stretch(double H, double s0, double sc, double s1, double k, double alpha, double denom)
+{
+    if (!(H > 0)) return s0;
+    if (H >= 1) return sc;
+    double lo = s0, hi = sc;
+    for (int it = 0; it < 64; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (law_factor<LAW>(mid, s0, sc, s1, k, alpha, denom) < H) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// calculatedamageenergy.m:47-57 at stretch s
+__device__ double damage_energy(double c, double s0, double sc, double L0, double s, double k, double alpha, double V)
+{
+    const double denominator = 2 * k * (exp(k) - 1) * (1 + alpha);
+    const double shape = (2 * k) - (2 * exp(k)) + (alpha * k) - (alpha * k * exp(k)) + 2;
+    double numerator;
+    if ((s0 < s) && (s <= sc)) numerator = c * s0 * L0 * (s0 - s) * shape;
+    else if (s > sc) numerator = c * s0 * L0 * (s0 - sc) * shape;
+    else return 0.0;
+    return (numerator / denominator) * (V * V);
+}
+
+template <bool MIXED, int LAW>
+__global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg, double *epart)
+{
+    __shared__ double sh[BLOCK];
+    const Region &RG = P.reg[ri];
+    const int n = (RG.blk0 + blockIdx.x) * BLOCK + threadIdx.x;
+    const bool real = n - RG.base < RG.nN;
+    const int cur = (int)(*P.tick & 1);
+    const NodeRec *pos = P.pos[cur];
+    double e_acc = 0;
+    int cnt = 0, unbroken = 0;
+    if (real) {
+        const NodeRec me = pos[n];
+        double Xn[3];
+        if (P.Xrec) {
+            Xn[0] = P.X0[n], Xn[1] = P.X1[n], Xn[2] = P.X2[n];
+        } else {
+            Xn[0] = P.hx * (double)(int)(me.q & LAT_MASK);
+            Xn[1] = P.hy * (double)(int)((me.q >> LAT_BITS) & LAT_MASK);
+            Xn[2] = P.hz * (double)(int)((me.q >> (2 * LAT_BITS)) & LAT_MASK);
+        }
+        auto bond = [&](double H, bool failed, int m, double c, double s0, double sc, double Xm0, double Xm1,
+                        double Xm2) {
+            const bool broken = failed || H == 1.0;
+            cnt++;
+            unbroken += broken ? 0 : 1;
+            if (m > n && LAW != BBPD_LAW_LINEAR) { // each bond once, at its lower slot
+                const double ex = Xn[0] - Xm0, ey = Xn[1] - Xm1, ez = Xn[2] - Xm2;
+                const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+                const double smax = max_stretch<LAW>(H, s0, sc, RG.s1, P.k, P.alpha, P.denom_exp);
+                e_acc += damage_energy(c, s0, sc, L0, smax, P.k, P.alpha, P.V);
+            }
+        };
+        if (P.stencil) {
+            const uint4 msk = P.smask[n];
+            const uint32_t mw[4] = {msk.x, msk.y, msk.z, msk.w};
+            const long long ebase = RG.e0 + (long long)((n - RG.base) >> 6) * RG.noff * SLICE + (n & (SLICE - 1));
+            const int gme = (int)(me.q >> G_SHIFT);
+            for (int o = 0; o < RG.noff; o++) {
+                if (!((mw[o >> 5] >> (o & 31)) & 1u)) continue;
+                const int4 so = P.soff[RG.off0 + o];
+                const long long e = ebase + (long long)o * SLICE;
+                const int m = n + so.x;
+                const double h = MIXED ? (double)reinterpret_cast<const float *>(P.hs)[e]
+                                       : reinterpret_cast<const double *>(P.hs)[e];
+                const bool sgn = signbit(h);
+                const int key = gme * RG.kg + (int)(pos[m].q >> G_SHIFT);
+                const double c = P.tab[RG.tab0 + key];
+                const double s0 = P.s0mode == PM_TABLE ? P.tab[RG.tab0 + RG.ts0 + key]
+                                  : (P.s0mode == PM_STREAM ? (MIXED ? (double)((const float *)P.s0str)[e]
+                                                                    : ((const double *)P.s0str)[e])
+                                                           : RG.s0);
+                const double sc = P.scmode == PM_TABLE ? P.tab[RG.tab0 + RG.tsc + key]
+                                  : (P.scmode == PM_STREAM ? (MIXED ? (double)((const float *)P.scstr)[e]
+                                                                    : ((const double *)P.scstr)[e])
+                                                           : RG.sc);
+                bond(fabs(h), P.law_linear && sgn, m, c, s0, sc, P.hx * (double)((int)(me.q & LAT_MASK) + so.y),
+                     P.hy * (double)((int)((me.q >> LAT_BITS) & LAT_MASK) + so.z),
+                     P.hz * (double)((int)((me.q >> (2 * LAT_BITS)) & LAT_MASK) + so.w));
+            }
+        } else {
+            const int len = P.rowlen[n];
+            long long e = P.slice_off[n >> 6] + (n & (SLICE - 1));
+            for (int kk = 0; kk < len; kk++, e += SLICE) {
+                const uint4 r = P.rec[e];
+                const uint32_t w = r.x;
+                const int m = (int)(w & NODE_MASK);
+                double H, c, s0, sc;
+                if (MIXED) {
+                    H = (double)__uint_as_float(r.z);
+                    s0 = P.s0mode == PM_INLINE ? (double)__uint_as_float(r.y)
+                         : (P.s0mode == PM_STREAM ? (double)((const float *)P.s0str)[e] : RG.s0);
+                    sc = P.scmode == PM_INLINE ? (double)__uint_as_float(r.y)
+                         : (P.scmode == PM_STREAM ? (double)((const float *)P.scstr)[e] : RG.sc);
+                    c = 0; // set below from c * V * vcf
+                } else {
+                    H = __hiloint2double((int)r.w, (int)r.z);
+                    c = P.cmode == PM_TABLE ? P.tab[r.y & 0xFFFFu] : P.cstr[e];
+                    s0 = P.s0mode == PM_TABLE ? P.tab[P.ntabc + (r.y >> 16)]
+                         : (P.s0mode == PM_STREAM ? ((const double *)P.s0str)[e] : RG.s0);
+                    sc = P.scmode == PM_TABLE ? P.tab[P.ntabc + (r.y >> 16)]
+                         : (P.scmode == PM_STREAM ? ((const double *)P.scstr)[e] : RG.sc);
+                }
+                double Xm[3];
+                if (P.Xrec) {
+                    Xm[0] = P.X0[m], Xm[1] = P.X1[m], Xm[2] = P.X2[m];
+                } else {
+                    const uint64_t q = pos[m].q;
+                    Xm[0] = P.hx * (double)(int)(q & LAT_MASK);
+                    Xm[1] = P.hy * (double)(int)((q >> LAT_BITS) & LAT_MASK);
+                    Xm[2] = P.hz * (double)(int)((q >> (2 * LAT_BITS)) & LAT_MASK);
+                }
+                if (MIXED) {
+                    // the mixed record holds c * V * vcf in FP32
+                    const double ex = Xn[0] - Xm[0], ey = Xn[1] - Xm[1], ez = Xn[2] - Xm[2];
+                    const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+                    double vcf;
+                    if (L0 <= P.hmr) vcf = 1;
+                    else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
+                    else vcf = 0;
+                    c = vcf > 0 ? (double)__uint_as_float(r.w) / (P.V * vcf) : 0.0;
+                }
+                bond(H, (w & FAILED_BIT) != 0, m, c, s0, sc, Xm[0], Xm[1], Xm[2]);
+            }
+        }
+        dmg[n - RG.base] = cnt > 0 ? 1 - ((double)unbroken / (double)cnt) : 0.0;
+    }
+    const double sum = bbpd::block_sum_fixed<BLOCK>(e_acc, sh);
+    if (threadIdx.x == 0) epart[blockIdx.x] = sum;
+}
+
 uint64_t morton3(uint32_t x, uint32_t y, uint32_t z)
 {
     auto spread = [](uint64_t v) {
@@ -2118,6 +2278,64 @@ int region_observables(bbpd_solver *s, int ri, bbpd_observables *o)
 
 } // namespace
 
+namespace {
+
+using DamageFn = void (*)(Params, int, double *, double *);
+
+template <bool MIXED>
+DamageFn pick_damage(int law)
+{
+    switch (law) {
+    case BBPD_LAW_LINEAR: return k_damage<MIXED, BBPD_LAW_LINEAR>;
+    case BBPD_LAW_BILINEAR: return k_damage<MIXED, BBPD_LAW_BILINEAR>;
+    case BBPD_LAW_TRILINEAR: return k_damage<MIXED, BBPD_LAW_TRILINEAR>;
+    default: return k_damage<MIXED, BBPD_LAW_EXPONENTIAL>;
+    }
+}
+
+// damage field (reference node order) and total damage energy of one region
+int region_damage(bbpd_solver *s, int ri, const std::vector<int> &slot_of, double *damage, double *energy)
+{
+    BBPD_HIP(hipSetDevice(s->device));
+    const Region &RG = s->reg[ri];
+    const bool mixed = s->precision == BBPD_PRECISION_MIXED;
+    double *dd = nullptr, *ep = nullptr;
+    BBPD_HIP(hipMalloc((void **)&dd, sizeof(double) * (size_t)RG.nSlots));
+    hipError_t e = hipMalloc((void **)&ep, sizeof(double) * (size_t)RG.nblk);
+    if (e != hipSuccess) {
+        (void)hipFree(dd);
+        return bbpd::hip_fail(e, "hipMalloc (damage)");
+    }
+    DamageFn f = mixed ? pick_damage<true>(s->law) : pick_damage<false>(s->law);
+    hipLaunchKernelGGL(f, dim3(RG.nblk), dim3(BLOCK), 0, s->stream, s->P, ri, dd, ep);
+    std::vector<double> hd(RG.nSlots), he(RG.nblk);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hd.data(), dd, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(he.data(), ep, sizeof(double) * he.size(), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dd);
+    (void)hipFree(ep);
+    if (e != hipSuccess) return bbpd::hip_fail(e, "damage kernel");
+    if (damage)
+        for (size_t n = 0; n < slot_of.size(); n++) damage[n] = hd[slot_of[n] - RG.base];
+    if (energy) {
+        double t = 0;
+        for (double x : he) t = t + x;
+        *energy = t;
+    }
+    return BBPD_OK;
+}
+
+} // namespace
+
+extern "C" int bbpd_solver_damage(bbpd_solver *s, double *damage, double *energy)
+{
+    BBPD_REQUIRE(s && !s->batch, "bad arguments (a batch takes bbpd_batch_damage)");
+    BBPD_REQUIRE(!energy || s->law != BBPD_LAW_LINEAR,
+                 "damage energy needs a softening law (the brittle latch keeps no stretch history)");
+    return region_damage(s, 0, s->slot_of, damage, energy);
+}
+
 extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
 {
     BBPD_REQUIRE(s && o, "null argument");
@@ -3180,4 +3398,12 @@ extern "C" int bbpd_batch_get_bonds(bbpd_solver *s, int region, double *H, doubl
         if (failout) failout[k] = (linear && sgn) ? 0.0 : 1.0;
     }
     return BBPD_OK;
+}
+
+extern "C" int bbpd_batch_damage(bbpd_solver *s, int region, double *damage, double *energy)
+{
+    BBPD_REQUIRE(s && s->batch && region >= 0 && region < s->maxRegions && s->active[region], "no such region");
+    BBPD_REQUIRE(!energy || s->law != BBPD_LAW_LINEAR,
+                 "damage energy needs a softening law (the brittle latch keeps no stretch history)");
+    return region_damage(s, region, s->rh[region].slot_of, damage, energy);
 }

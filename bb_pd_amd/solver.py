@@ -118,6 +118,14 @@ class Solver:
         return dict(reaction=r[:, 0].copy(), reaction_static=r[:, 1].copy(), deflection=r[:, 2].copy(),
                     cmod=r[:, 3].copy())
 
+    def damage(self, energy: bool = True):
+        """(damage field per node in the reference order, total damage energy or None)."""
+        d = np.zeros(self.case.nNodes)
+        e = C.c_double()
+        _lib.check(self._L.bbpd_solver_damage(self._h, d.ctypes.data_as(C.c_void_p), C.byref(e) if energy else None),
+                   "bbpd_solver_damage")
+        return d, (e.value if energy else None)
+
     def layout(self) -> str:
         """"stencil" or "records (<reason>)": the bond-table layout chosen at set-up."""
         return self._L.bbpd_solver_layout(self._h).decode()

@@ -236,6 +236,15 @@ int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u, const dou
  * body-0 family), deflection u_z(referenceNode), CMOD u_x(cmod2) - u_x(cmod1).  At most `cap` steps
  * are returned (the device keeps the last 2^20). */
 int bbpd_solver_history(bbpd_solver *s, double *rows, int cap, int *n_out);
+/* Damage field (calculatedamage.m:34-45: 1 - unbroken / nFAMILY per node, broken meaning H == 1 or
+ * failed; damage is nNodes, host, reference order, may be NULL) and total dissipated damage energy
+ * (calculatedamageenergy.m:35-62 summed over bonds, with the solver's k and alpha; may be NULL).
+ * The reference keeps each bond's energy as a running max over steps; here it is evaluated from
+ * the largest stretch the bond has reached, which its softening history fixes (the law inverted at
+ * H).  That equals the running max for every loading path except the bilinear law's compression
+ * branch, where the reference sets H = 1 without the stretch passing sc.  Energy needs a softening
+ * law. */
+int bbpd_solver_damage(bbpd_solver *s, double *damage, double *energy);
 /* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */
 int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel_bytes, int64_t *n_entries,
                      int64_t *n_padded_entries);
@@ -303,6 +312,7 @@ int bbpd_batch_get_nodal(bbpd_solver *s, int region, double *nodalDisplacement, 
                          double *nodalForce, double *deformedCoordinates);
 int bbpd_batch_get_bonds(bbpd_solver *s, int region, double *bondSofteningFactor, double *flagBondSoftening,
                          double *fail);
+int bbpd_batch_damage(bbpd_solver *s, int region, double *damage, double *energy);
 
 #ifdef __cplusplus
 }

@@ -425,3 +425,32 @@ def test_autotune_leaves_the_trajectory_unchanged(gpu):
     with Solver(ec_case(_lib.LAW_BILINEAR), _lib.MIXED) as e:
         with pytest.raises(_lib.BBPDError, match="ADR"):
             e.autotune(3)
+
+
+@pytest.mark.parametrize("law", [_lib.LAW_EXPONENTIAL, _lib.LAW_TRILINEAR])
+def test_damage_field_and_energy_match_oracle(gpu, oracle, law):
+    """Device damage field and damage energy against the oracle, which keeps the reference's per-bond
+    running max of calculatedamageenergy over every step (dynamicsolverdisplacementcontrolled.m:121)."""
+    case = ec_case(law)
+    run = oracle.OracleRun(case)
+    hist = np.zeros(case.nBonds)
+    s0 = np.ascontiguousarray(run.s0_full)
+    sc = np.ascontiguousarray(run.sc_full)
+    for _ in range(300):
+        run.step(1)
+        oracle.calculatedamageenergy(case.c, s0, sc, case.L0, run.s, case.k, case.alpha, case.cellVolume, hist)
+    e_ref = hist.sum()
+    assert e_ref > 0
+    unbroken = (~((run.H == 1) | (run.fail == 0))).astype(np.float64)
+    d_ref = oracle.calculatedamage(case.BONDLIST, unbroken, case.nFAM)
+    for layout, prec, tol in (("records", _lib.FP64, 1e-9), ("stencil", _lib.FP64, 1e-9),
+                              ("stencil", _lib.MIXED, 2e-3)):
+        with _solver_with_layout(case, prec, layout) as sol:
+            sol.step(300)
+            d, e = sol.damage()
+            print(f"{layout} {prec}: energy {e:.6e} vs {e_ref:.6e}, damage max {d.max():.3f}")
+            assert e == pytest.approx(e_ref, rel=tol)
+            if prec == _lib.FP64:
+                assert np.array_equal(d, d_ref)
+            else:
+                assert np.abs(d - d_ref).max() <= 0.05
