@@ -117,18 +117,9 @@ def increment_curve(R: np.ndarray, ipi: int) -> np.ndarray:
     return np.array([R[i * ipi + ipi - h:(i + 1) * ipi].mean() for i in range(n)])
 
 
-def damage_field(case, H, fail):
-    """calculatedamage.m:34-45 with broken = (bondSofteningFactor == 1) or (fail == 0)."""
-    unbroken = ~((H == 1) | (fail == 0))
-    i = case.BONDLIST[:, 0].astype(np.int64) - 1
-    j = case.BONDLIST[:, 1].astype(np.int64) - 1
-    cnt = np.bincount(i, weights=unbroken, minlength=case.nNodes) + np.bincount(j, weights=unbroken,
-                                                                              minlength=case.nNodes)
-    return 1 - cnt / case.nFAM
-
-
 def one_step_from(sol, st):
-    """Restore a checkpoint, run one step, return (nodal force, static reaction, H, flag, fail)."""
+    """Restore a checkpoint, run one step, return (nodal force, static reaction, H, flag, fail,
+    damage field, damage energy); the damage observables are computed on the device."""
     sol.history()
     sol.restore(st["step"], **{k: v for k, v in st.items() if k != "step"})
     sol.step(1)
@@ -136,7 +127,8 @@ def one_step_from(sol, st):
     R = float(sol.history()["reaction_static"][-1])
     F = sol.nodal()[2]
     H, flag, fail = sol.bonds()
-    return F, R, H, flag, fail
+    D, W = sol.damage()
+    return F, R, H, flag, fail, D, W
 
 
 def precision_study(case, fp64, mixed, ipi):
@@ -167,15 +159,15 @@ def precision_study(case, fp64, mixed, ipi):
     same = []
     for k in sorted(ck):
         st = ck[k]
-        Fa, Ra, Ha, fla, faa = one_step_from(A, st)
-        Fb, Rb, Hb, flb, fab = one_step_from(B, st)
-        Da, Db = damage_field(case, Ha, faa), damage_field(case, Hb, fab)
+        Fa, Ra, Ha, fla, faa, Da, Wa = one_step_from(A, st)
+        Fb, Rb, Hb, flb, fab, Db, Wb = one_step_from(B, st)
         same.append(dict(step=k, load=Ra, load_err=(Rb - Ra) / Ra if Ra else None,
                          nodal_force_err=float(np.abs(Fb - Fa).max() / np.abs(Fa).max()),
                          flag_flips=int((fla != flb).sum()),
                          broken_fp64=int(((Ha == 1) | (faa == 0)).sum()),
                          broken_mixed=int(((Hb == 1) | (fab == 0)).sum()),
-                         damage_err=float(np.abs(Db - Da).max())))
+                         damage_err=float(np.abs(Db - Da).max()),
+                         damage_energy=Wa, damage_energy_err=(Wb - Wa) / Wa if Wa else None))
         log(f"  same state at step {k}: {same[-1]}")
     # rounding floor: restart FP64 after increment 1 with one ulp on the largest deflection
     st = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in ck[ipi].items()}
