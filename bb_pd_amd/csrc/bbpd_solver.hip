@@ -58,6 +58,8 @@ constexpr uint64_t LAT_MASK = (1ull << LAT_BITS) - 1;
 constexpr int G_SHIFT = 56;    // node record: the node's bond count (nFAMILYMEMBERS) in the top byte
 constexpr int ST_MAX = 128;    // offsets of the stencil layout (bits of the per-node mask)
 constexpr int ST_GMAX = 45;    // distinct family sizes of the stencil layout (key table <= 45^2)
+constexpr int ST_RMAX = 15;    // largest |dx| of a stencil offset
+constexpr int ST_ROW = 96;     // sites of a staged stencil row: 64 + 2 * ST_RMAX, rounded up
 // node-flag bits 0-2: DOF d constrained; bit 3: member of a contact family (its DOFs are driven by
 // the rigid-body projection, so the ADR Rayleigh quotient leaves the node out)
 constexpr unsigned char CONTACT_BIT = 1u << 3;
@@ -75,6 +77,7 @@ struct Region {
     int blk0, nblk;                // global blocks [blk0, blk0 + nblk)
     long long e0;                  // stencil layout: first entry of the region
     int noff, off0;                // stencil layout: offsets off0 .. off0 + noff - 1
+    int nrow, row0, rx;            // stencil layout: rows row0 .. row0 + nrow - 1, largest |dx|
     int kg, tab0, ntab, ts0, tsc;  // stencil layout: key tables (ts0 / tsc relative to tab0)
     double s0, s1, sc;             // scalar law parameters
     int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
@@ -156,6 +159,7 @@ struct Params {
     const int4 *soff;     // per offset: slot delta, dx, dy, dz
     const float4 *sxi;    // per offset (mixed form): xi = h * d in FP32, |xi|
     const float *svcf;    // per offset (mixed form): volume correction factor of |h * d|
+    const int2 *srow;     // per stencil row (offsets sharing dy, dz): offset count, slot delta at dx = 0
     const uint4 *smask;   // per slot: bit o set when the node has the bond at offset o
     void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
     const float *tabf;    // mixed-form key tables (c * V, then parameter tables)
@@ -332,6 +336,15 @@ __device__ inline int launch_block(const Params &P, bool remap = true)
 // reverse for k+1.  The loop runs to the slice width (uniform over the wavefront); padding entries
 // are valid memory, so every load is unconditional and only the arithmetic past the row is masked.
 // ------------------------------------------------------------------------------------------------
+// Mixed form: the factor p of the bond force p * (xi + eta).  The reference zeroes NaN force
+// components, which a zero deformed length produces (calculatebondforcesMex.c:38); with finite
+// positions the factor is non-finite only then (or when it overflows), so one test of the factor
+// stands for the three tests of the components.
+__device__ __forceinline__ float mixed_force_factor(float p)
+{
+    return fabsf(p) <= 3.402823466e38f ? p : 0.0f;
+}
+
 template <typename T>
 struct Ent {
     uint4 r;             // entry record
@@ -461,14 +474,10 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             apply_law<LAW, float>(s, s0, sc, (float)rs1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
                                   (float)P.sc_steel, (float)P.ff, H, w);
             const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
-            const float p = fail * kap * (1.0f - H) * s * rl;
-            float gx = p * y0, gy = p * y1, gz = p * y2;
-            if (gx != gx) gx = 0.0f;
-            if (gy != gy) gy = 0.0f;
-            if (gz != gz) gz = 0.0f;
-            f0 += (double)gx;
-            f1 += (double)gy;
-            f2 += (double)gz;
+            const float p = mixed_force_factor(fail * kap * (1.0f - H) * s * rl);
+            f0 += (double)(p * y0);
+            f1 += (double)(p * y1);
+            f2 += (double)(p * y2);
             if (__float_as_uint(H) != __float_as_uint(H0)) *rec_word(rec, e, 2) = __float_as_uint(H);
         }
         if (w != d.r.x) *rec_word(rec, e, 0) = w;
@@ -504,24 +513,105 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 struct StEnt {
-    T h;        // stored history (sign = flag)
-    T p0, p1;   // streamed s0, sc
-    NodeRec nb; // neighbour record
+    T h;      // stored history (sign = flag)
+    T p0, p1; // streamed s0, sc
+    // per-offset data (the same for every lane): mixed form xi, vcf and dx; FP64 form the offset
+    float4 xi;
+    float vcf;
+    int dx;
+    int4 so;
 };
+
+// one stencil row of neighbour records in flight: sites 0 .. 63 (a) and 64 .. 63 + 2 rx (b)
+struct RowRecs {
+    NodeRec a, b;
+};
+
+// Per-offset tables are the same for every lane of a wavefront and no kernel writes them: read
+// through the constant address space, they become scalar loads into scalar registers.
+template <typename T>
+using sconst_ptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ T sload(const T *p, int i)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 16, "32-bit or 128-bit elements");
+    if constexpr (sizeof(T) == 16)
+        return __builtin_bit_cast(T, ((sconst_ptr<u32x4>)p)[i]);
+    else
+        return __builtin_bit_cast(T, ((sconst_ptr<uint32_t>)p)[i]);
+}
+
+// Buffer resources (gfx9 descriptor: 32-bit data format, raw addressing).  The address unit adds a
+// scalar base, a per-lane 32-bit offset and a scalar offset, so a stream whose per-lane offset is
+// fixed costs no vector instruction per access; a read at a per-lane offset past num_records
+// returns zeros without touching memory.
+using brsrc = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int CP_NT = 2; // cache policy: non-temporal
+
+__device__ __forceinline__ brsrc make_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T bload_nt(brsrc r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, CP_NT));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CP_NT));
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(T v, brsrc r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+struct RecWords {
+    u32x4 a, b;
+};
+
+__device__ __forceinline__ NodeRec bload_rec(brsrc r, uint32_t voff)
+{
+    RecWords w;
+    w.a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+    w.b = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16u, 0, 0);
+    return __builtin_bit_cast(NodeRec, w);
+}
+
+// True when the law can change the history H or the flag at stretch s.  Every value a law stores
+// lies in [0, 1] (calculatebsfexponentialMex.c:55-58 clamps; the others are maxima of values in
+// [0, 1]) and set-up and restore accept only such values; with BONDTYPE 0 (the stencil layout's
+// only type), a stretch at or below the law's lowest threshold then leaves H and the flag as they
+// are (calculatebondfailureMex.c:36-38, calculatebondsofteningfactor.m:42-48,
+// calculatebsftrilinear.m:80-112, calculatebsfexponentialMex.c:35-58).  A bilinear bond, once
+// flagged, is evaluated every step: in compression its candidate is positive.
+template <int LAW, typename T>
+__device__ __forceinline__ bool law_active(T s, T s0, T sc, T s1, bool flag)
+{
+    if (LAW == BBPD_LAW_LINEAR) return s > sc;
+    if (LAW == BBPD_LAW_BILINEAR) return flag || s > s0;
+    if (LAW == BBPD_LAW_TRILINEAR) return s > fmin(fmin(s0, s1), sc);
+    return s > fmin(s0, sc);
+}
 
 // Launch variants, picked per solver by an autotuner: OCC is the minimum waves per SIMD the compiler
 // must allow (1 = its own choice), which bounds the registers per wave, trading instruction-level
-// for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.
-template <bool MIXED, int LAW, int OCC, bool REMAP>
+// for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.  GEN takes
+// s0 and sc from key tables or per-entry streams; without it they are the region's scalars.
+template <bool MIXED, int LAW, int OCC, bool REMAP, bool GEN>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) void k_bond_st(Params P)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
+    constexpr uint32_t TW = sizeof(T);
     __shared__ double sh[BLOCK];
-    // per-offset data in LDS: a neighbour address depends on its offset's slot delta, and an LDS
-    // read is far shorter than the global load it would otherwise wait for
-    __shared__ int4 s_off[ST_MAX];
-    __shared__ float4 s_xi[MIXED ? ST_MAX : 1];
-    __shared__ float s_vcf[MIXED ? ST_MAX : 1];
+    // per wavefront, two row buffers: x, y, z and the family-size index (low word) of ST_ROW sites
+    __shared__ double s_row[2][BLOCK / SLICE][4 * ST_ROW];
     extern __shared__ double tabs[]; // key tables (FP64, or FP32 in the mixed form)
     const int gb = launch_block(P, REMAP);
     if (gb < 0) return;
@@ -529,9 +619,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const Region &RG = P.reg[P.blkreg[gb]];
     const int base = RG.base, noff = RG.noff, off0 = RG.off0, kg = RG.kg, tab0 = RG.tab0, ntab = RG.ntab;
     const int ts0 = RG.ts0, tsc = RG.tsc;
-    const long long e0 = RG.e0;
     const bool real = n - base < RG.nN;
-    const double rs0 = RG.s0, rs1 = RG.s1, rsc = RG.sc;
+    const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc;
+    const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
     const int cur = (int)(*P.tick & 1);
     const NodeRec *__restrict__ pos = P.pos[cur];
     if constexpr (MIXED) {
@@ -539,13 +629,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         for (int i = threadIdx.x; i < ntab; i += BLOCK) tf[i] = P.tabf[tab0 + i];
     } else {
         for (int i = threadIdx.x; i < ntab; i += BLOCK) tabs[i] = P.tab[tab0 + i];
-    }
-    for (int i = threadIdx.x; i < noff; i += BLOCK) {
-        s_off[i] = P.soff[off0 + i];
-        if constexpr (MIXED) {
-            s_xi[i] = P.sxi[off0 + i];
-            s_vcf[i] = P.svcf[off0 + i];
-        }
     }
     __syncthreads();
     const T *tab = reinterpret_cast<const T *>(tabs);
@@ -558,51 +641,116 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     // X == h * k exactly (checked at set-up)
     const double Xn0 = P.hx * (double)kx, Xn1 = P.hy * (double)ky, Xn2 = P.hz * (double)kz;
     const uint4 msk = P.smask[n];
-    // the bond mask, shifted right as the loop advances: bit 0 is the offset being computed
-    unsigned __int128 mk = ((unsigned __int128)(((uint64_t)msk.w << 32) | msk.z) << 64) |
-                           (((uint64_t)msk.y << 32) | msk.x);
-    T *__restrict__ hs = reinterpret_cast<T *>(P.hs);
-    const long long ebase = e0 + (long long)((n - base) >> 6) * noff * SLICE + (n & (SLICE - 1));
-    double f0 = 0, f1 = 0, f2 = 0;
-    // `bit` = this lane has the bond at offset o
-    auto load = [&](StEnt<T> &d, int o, bool bit) {
-        o = min(o, noff - 1);
-        const long long e = ebase + (long long)o * SLICE;
-        const int m = bit ? n + s_off[o].x : n; // absent bonds read the node itself (valid memory)
-        d.h = __builtin_nontemporal_load(hs + e);
-        if (P.s0mode == PM_STREAM) d.p0 = __builtin_nontemporal_load((const T *)P.s0str + e);
-        if (P.scmode == PM_STREAM) d.p1 = __builtin_nontemporal_load((const T *)P.scstr + e);
-        d.nb = pos[m];
+    // this wavefront's slice of the entry arrays: entry (o, lane) at byte (o * SLICE + lane) * TW
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / SLICE));
+    const long long eslice = RG.e0 + ((long long)((gb * BLOCK - base) / SLICE) + wv) * noff * SLICE;
+    const uint32_t sbytes = (uint32_t)noff * SLICE * TW;
+    const brsrc rh = make_rsrc((const T *)P.hs + eslice, sbytes);
+    const brsrc r0 = GEN && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
+    const brsrc r1 = GEN && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
+    const uint32_t lo = (threadIdx.x % SLICE) * TW;
+    // Neighbour records, staged per stencil row (the offsets that share dy and dz, consecutive in
+    // offset order).  For the slice starting at slot n0, row r covers the slots
+    // n0 + delta_r - rx .. n0 + delta_r + 63 + rx; the wavefront loads them once into its LDS row
+    // buffer, and the offset (dx, dy, dz) of lane i reads site i + rx + dx there.  The next row's
+    // loads are in flight while a row is computed.  Slot offsets are 32-bit: a site before slot 0
+    // wraps to an offset past every valid one (set-up keeps slots below 2^26), and a read outside
+    // the slot array returns zeros (such a site is never a bond).
+    const brsrc rp = make_rsrc(pos, (uint32_t)P.nSlots * (uint32_t)sizeof(NodeRec));
+    const int lane = (int)(threadIdx.x % SLICE);
+    const int n0 = gb * BLOCK + wv * SLICE;
+    const int rx = RG.rx, nrow = RG.nrow;
+    const int *rows = reinterpret_cast<const int *>(P.srow + RG.row0); // (count, delta) per row
+    const int4 *offs = P.soff + off0;
+    const int *dls = reinterpret_cast<const int *>(offs); // offset o: dls[4 * o] = delta, dls[4 * o + 1] = dx
+    auto fetch = [&](RowRecs &R, int r) {
+        const uint32_t sb = (uint32_t)(n0 + sload(rows, 2 * r + 1) - rx);
+        R.a = bload_rec(rp, (sb + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
+        if (lane < 2 * rx) R.b = bload_rec(rp, (sb + SLICE + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
     };
-    auto compute = [&](const StEnt<T> &d, int o, bool bit) {
+    auto put = [&](double *w, int site, const NodeRec &q) {
+        w[site] = q.x;
+        w[ST_ROW + site] = q.y;
+        w[2 * ST_ROW + site] = q.z;
+        *reinterpret_cast<int *>(w + 3 * ST_ROW + site) = (int)(q.q >> G_SHIFT);
+        // the record's low lattice word is not used; keeping it live until here stops the register
+        // allocator from reusing its register while the load is in flight (which would make the
+        // wavefront wait for every outstanding load)
+        __asm__ volatile("" ::"v"((uint32_t)q.q));
+    };
+    auto stage = [&](const RowRecs &R, int buf) {
+        double *w = &s_row[buf][wv][0];
+        put(w, lane, R.a);
+        if (lane < 2 * rx) put(w, SLICE + lane, R.b);
+    };
+    double f0 = 0, f1 = 0, f2 = 0;
+    auto load = [&](StEnt<T> &d, int o) {
+        o = min(o, noff - 1);
+        const uint32_t so = (uint32_t)o * (SLICE * TW);
+        d.h = bload_nt<T>(rh, lo, so);
+        if (GEN && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lo, so);
+        if (GEN && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lo, so);
+        if constexpr (MIXED) {
+            d.xi = sload(P.sxi + off0, o);
+            d.vcf = sload(P.svcf + off0, o);
+            d.dx = sload(dls, 4 * o + 1);
+        } else {
+            d.so = sload(offs, o);
+            d.dx = d.so.y;
+        }
+    };
+    auto compute = [&](const StEnt<T> &d, int o, bool bit, int buf) {
         if (!bit) return;
-        const long long e = ebase + (long long)o * SLICE;
-        const int key = gme * kg + (int)(d.nb.q >> G_SHIFT);
-        const bool sgn = MIXED ? (__float_as_uint((float)d.h) >> 31) != 0u
-                               : (__double_as_longlong((double)d.h) < 0);
-        const uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
-        uint32_t w = sgn ? mark : 0u;
-        T H = d.h < T(0) ? -d.h : d.h; // |h|, and -0 -> +0
-        if (H == T(0)) H = T(0);
-        const T s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : (T)rs0);
-        const T sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : (T)rsc);
+        const uint32_t so = (uint32_t)o * (SLICE * TW);
+        const double *nb = &s_row[buf][wv][0] + (lane + rx + d.dx);
+        const double nbx = nb[0], nby = nb[ST_ROW], nbz = nb[2 * ST_ROW];
+        const int key = gme * kg + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
+        constexpr uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
+        // the history is |h| (+0 for -0), the flag its sign bit
+        T H;
+        bool flag;
+        if constexpr (MIXED) {
+            const uint32_t hb = __float_as_uint(d.h);
+            H = __uint_as_float(hb & 0x7FFFFFFFu);
+            flag = (int)hb < 0;
+        } else {
+            const unsigned long long hb = (unsigned long long)__double_as_longlong(d.h);
+            H = __longlong_as_double((long long)(hb & 0x7FFFFFFFFFFFFFFFull));
+            flag = (long long)hb < 0;
+        }
+        T s0 = rs0, sc = rsc;
+        if (GEN) {
+            s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : rs0);
+            sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : rsc);
+        }
+        // the law, and the store of a changed history (sign = flag)
+        auto law = [&](T s) {
+            if (!law_active<LAW, T>(s, s0, sc, rs1, flag)) return;
+            uint32_t w = flag ? mark : 0u;
+            apply_law<LAW, T>(s, s0, sc, rs1, lk, lalpha, ldenom, lsteel, lff, H, w);
+            flag = (w & mark) != 0u;
+            const T hn = flag ? -H : H;
+            if (__builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, hn) !=
+                __builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, d.h))
+                bstore<T>(hn, rh, lo, so);
+        };
         if constexpr (!MIXED) {
-            const int4 so = s_off[o];
-            const double Xm0 = P.hx * (double)(kx + so.y), Xm1 = P.hy * (double)(ky + so.z),
-                         Xm2 = P.hz * (double)(kz + so.w);
+            const int4 sof = d.so;
+            const double Xm0 = P.hx * (double)(kx + sof.y), Xm1 = P.hy * (double)(ky + sof.z),
+                         Xm2 = P.hz * (double)(kz + sof.w);
             const double c = tab[key];
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
-            const double dx = d.nb.x - qn0, dy = d.nb.y - qn1, dz = d.nb.z - qn2;
+            const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
             const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
             const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
             const double s = (l - L0) / L0;
-            apply_law<LAW, double>(s, s0, sc, rs1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
+            law(s);
             double vcf; // calculatevolumecorrectionfactors.m:20-32
             if (L0 <= P.hmr) vcf = 1;
             else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
             else vcf = 0;
-            const double fail = (w & FAILED_BIT) ? 0.0 : 1.0;
+            const double fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0 : 1.0;
             const double p = fail * c * (1 - H) * s * P.V * vcf;
             // correctly rounded quotients dx/l etc. (see k_bond)
             const double rl = 1.0 / l;
@@ -617,12 +765,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             f0 += gx;
             f1 += gy;
             f2 += gz;
-            const double hn = (w & mark) ? -H : H;
-            if (__double_as_longlong(hn) != __double_as_longlong((double)d.h)) hs[e] = hn;
         } else {
-            const float4 xi = s_xi[o];
-            const float kap = tab[key] * s_vcf[o];
-            const float et0 = (float)(d.nb.x - qn0), et1 = (float)(d.nb.y - qn1), et2 = (float)(d.nb.z - qn2);
+            const float4 xi = d.xi;
+            const float kap = tab[key] * d.vcf;
+            const float et0 = (float)(nbx - qn0), et1 = (float)(nby - qn1), et2 = (float)(nbz - qn2);
             const float y0 = xi.x + et0, y1 = xi.y + et1, y2 = xi.z + et2;
             const float L0 = xi.w;
             const float l = __builtin_amdgcn_sqrtf(fmaf(y0, y0, fmaf(y1, y1, y2 * y2)));
@@ -630,31 +776,44 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
                 fmaf(2.0f, fmaf(xi.x, et0, fmaf(xi.y, et1, xi.z * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
             const float rl = __builtin_amdgcn_rcpf(l);
             const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
-            apply_law<LAW, float>(s, s0, sc, (float)rs1, (float)P.k, (float)P.alpha, (float)P.denom_exp,
-                                  (float)P.sc_steel, (float)P.ff, H, w);
-            const float fail = (w & FAILED_BIT) ? 0.0f : 1.0f;
-            const float p = fail * kap * (1.0f - H) * s * rl;
-            float gx = p * y0, gy = p * y1, gz = p * y2;
-            if (gx != gx) gx = 0.0f;
-            if (gy != gy) gy = 0.0f;
-            if (gz != gz) gz = 0.0f;
-            f0 += (double)gx;
-            f1 += (double)gy;
-            f2 += (double)gz;
-            const float hn = (w & mark) ? -H : H;
-            if (__float_as_uint(hn) != __float_as_uint((float)d.h)) hs[e] = hn;
+            law(s);
+            const float fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0f : 1.0f;
+            const float p = mixed_force_factor(fail * kap * (1.0f - H) * s * rl);
+            f0 += (double)(p * y0);
+            f1 += (double)(p * y1);
+            f2 += (double)(p * y2);
         }
     };
-    {
+    if (noff > 0) {
+        RowRecs R;
+        fetch(R, 0);
+        stage(R, 0);
+        int r = 0, rend = sload(rows, 0);
+        if (nrow > 1) fetch(R, 1);
+        // entering offset o: at a row's end, stage the next row and start loading the one after
+        auto next_row = [&](int o) {
+            if (o == rend) {
+                r++;
+                stage(R, r & 1);
+                rend += sload(rows, 2 * r);
+                if (r + 1 < nrow) fetch(R, r + 1);
+            }
+        };
         StEnt<T> A, B;
-        load(A, 0, (uint32_t)mk & 1u);
-        for (int o = 0; o < noff; o += 2) {
-            const uint32_t b3 = (uint32_t)mk & 7u; // offsets o, o + 1, o + 2
-            load(B, o + 1, (b3 & 2u) != 0u);
-            compute(A, o, (b3 & 1u) != 0u);
-            load(A, o + 2, (b3 & 4u) != 0u);
-            compute(B, o + 1, (b3 & 2u) != 0u);
-            mk >>= 2;
+        load(A, 0);
+        for (int c = 0; c < noff; c += 32) {
+            // bond-mask bits of offsets c .. c + 31, shifted as o advances
+            uint32_t mw = c == 0 ? msk.x : (c == 32 ? msk.y : (c == 64 ? msk.z : msk.w));
+            const int oe = min(c + 32, noff);
+            for (int o = c; o < oe; o += 2) {
+                load(B, o + 1);
+                next_row(o);
+                compute(A, o, (mw & 1u) != 0u, r & 1);
+                load(A, o + 2);
+                next_row(o + 1);
+                compute(B, o + 1, (mw & 2u) != 0u, r & 1);
+                mw >>= 2;
+            }
         }
     }
     finish_node<MIXED>(P, n, gb, real, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
@@ -1227,25 +1386,25 @@ KernelFn pick_law(int law, bool lat)
 // launch variants of the stencil bond kernel: (minimum waves per SIMD, XCD block remap)
 constexpr int N_OCC = 4;
 
-template <bool MIXED, int LAW>
+template <bool MIXED, int LAW, bool GEN>
 KernelFn pick_occ(int v)
 {
     switch (v) {
-    case 1: return k_bond_st<MIXED, LAW, 1, false>;
-    case 2: return k_bond_st<MIXED, LAW, 5, true>;
-    case 3: return k_bond_st<MIXED, LAW, 8, true>;
-    default: return k_bond_st<MIXED, LAW, 1, true>;
+    case 1: return k_bond_st<MIXED, LAW, 1, false, GEN>;
+    case 2: return k_bond_st<MIXED, LAW, 5, true, GEN>;
+    case 3: return k_bond_st<MIXED, LAW, 8, true, GEN>;
+    default: return k_bond_st<MIXED, LAW, 1, true, GEN>;
     }
 }
 
-template <bool MIXED>
+template <bool MIXED, bool GEN>
 KernelFn pick_law_st(int law, int v)
 {
     switch (law) {
-    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR>(v);
-    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR>(v);
-    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR>(v);
-    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL>(v);
+    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR, GEN>(v);
+    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR, GEN>(v);
+    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR, GEN>(v);
+    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL, GEN>(v);
     }
 }
 
@@ -1253,8 +1412,24 @@ KernelFn pick_bond_kernel(const bbpd_solver *s, int variant = -1)
 {
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     const int v = variant >= 0 ? variant : s->occ;
-    if (s->stencil) return mixed ? pick_law_st<true>(s->law, v) : pick_law_st<false>(s->law, v);
+    if (s->stencil) {
+        // s0 and sc as the region's scalars, or from key tables / per-entry streams
+        const bool gen = s->P.s0mode != PM_SCALAR || s->P.scmode != PM_SCALAR;
+        if (mixed) return gen ? pick_law_st<true, true>(s->law, v) : pick_law_st<true, false>(s->law, v);
+        return gen ? pick_law_st<false, true>(s->law, v) : pick_law_st<false, false>(s->law, v);
+    }
     return mixed ? pick_law<true>(s->law, s->lattice) : pick_law<false>(s->law, s->lattice);
+}
+
+// Let every launch variant of the solver's bond kernel take its key tables in dynamic LDS beside
+// its static LDS (the stencil kernel's row buffers), past the default per-block limit if needed.
+int allow_lds(bbpd_solver *s)
+{
+    if (s->lds_bytes == 0) return BBPD_OK;
+    for (int v = 0; v < (s->stencil ? N_OCC : 1); v++)
+        BBPD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(pick_bond_kernel(s, v)),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->lds_bytes));
+    return BBPD_OK;
 }
 
 int launch_step(bbpd_solver *s, bool allow_timing)
@@ -1419,6 +1594,8 @@ struct StencilPlan {
     std::string why;              // reason the layout does not apply
     int noff = 0;                 // offsets, padded to even
     std::vector<int4> off;        // slot delta, dx, dy, dz
+    std::vector<int2> rows;       // offset count and slot delta of dx = 0, per run of equal (dy, dz)
+    int rx = 0;                   // largest |dx|
     std::vector<int> slot;        // node -> slot
     std::vector<int> oi, oj;      // per bond: offset index at the i end and at the j end
     std::vector<int> g;           // per node: bond count
@@ -1466,6 +1643,10 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
         sp.why = "the nodes do not fill a lattice box";
         return;
     }
+    if ((long long)nN + SLOT_ALIGN > (1LL << 26)) {
+        sp.why = "more than 2^26 nodes (node-record offsets are 32-bit)";
+        return;
+    }
     sp.slot.assign(nN, 0);
     std::vector<char> seen(nN, 0);
     for (int n = 0; n < nN; n++) {
@@ -1492,7 +1673,7 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
     int R = 0;
     for (int b = 0; b < nB; b++)
         for (int d = 0; d < 3; d++) R = std::max(R, std::abs(sp.k[3 * (size_t)bj[b] + d] - sp.k[3 * (size_t)bi[b] + d]));
-    if (R > 15) {
+    if (R > ST_RMAX) {
         sp.why = "bond offsets longer than 15 lattice spacings";
         return;
     }
@@ -1518,8 +1699,19 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
         sp.why = "more than 128 distinct bond offsets";
         return;
     }
-    if (no & 1) sp.off.push_back(make_int4(0, 0, 0, 0)); // padding offset, never a bond
+    // padding offset, never a bond: a copy of the last, so it extends the last row
+    if (no & 1) sp.off.push_back(sp.off.back());
     sp.noff = (int)sp.off.size();
+    // rows: runs of offsets with equal (dy, dz), with the slot delta of their dx = 0 site
+    sp.rows.clear();
+    sp.rx = 0;
+    for (int o = 0; o < sp.noff;) {
+        int e = o;
+        while (e < sp.noff && sp.off[e].z == sp.off[o].z && sp.off[e].w == sp.off[o].w) e++;
+        sp.rows.push_back(make_int2(e - o, sp.off[o].x - sp.off[o].y));
+        o = e;
+    }
+    for (const int4 &q : sp.off) sp.rx = std::max(sp.rx, std::abs(q.y));
     // rows in ascending offset order, each offset at most once per node
     sp.oi.assign(nB, 0);
     sp.oj.assign(nB, 0);
@@ -1844,12 +2036,14 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     void *hsd = nullptr;
     uint4 *maskd = nullptr;
     int4 *stoffd = nullptr;
+    int2 *srowd = nullptr;
     float4 *sxid = nullptr;
     float *svcfd = nullptr, *tabfd = nullptr;
     if (sp.ok) {
         if ((rc = dalloc((char **)&hsd, (size_t)nE * (mixed ? 4 : 8), s->owned))) return fail(rc);
         if ((rc = dalloc(&maskd, nSlots, s->owned))) return fail(rc);
         if ((rc = dalloc(&stoffd, sp.noff, s->owned))) return fail(rc);
+        if ((rc = dalloc(&srowd, sp.rows.size(), s->owned))) return fail(rc);
         if ((rc = dalloc(&sxid, sp.noff, s->owned))) return fail(rc);
         if ((rc = dalloc(&svcfd, sp.noff, s->owned))) return fail(rc);
     }
@@ -1942,6 +2136,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
             BBPD_HIP(hipMemcpy(hsd, h0.data(), sizeof(double) * nE, hipMemcpyHostToDevice));
         }
         BBPD_HIP(hipMemcpy(stoffd, sp.off.data(), sizeof(int4) * sp.noff, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(srowd, sp.rows.data(), sizeof(int2) * sp.rows.size(), hipMemcpyHostToDevice));
         // mixed form per offset: xi = h * d rounded to FP32 (so the offset and its negation give
         // exactly negated vectors), |xi|, and the volume correction factor of |h * d|
         std::vector<float4> hx(sp.noff);
@@ -2031,12 +2226,14 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.partials = partd;
     P.stencil = sp.ok ? 1 : 0;
     P.soff = stoffd;
+    P.srow = srowd;
     P.sxi = sxid;
     P.svcf = svcfd;
     P.smask = maskd;
     P.hs = hsd;
     P.tabf = tabfd;
     s->lds_bytes = sp.ok ? (mixed ? sizeof(float) : sizeof(double)) * ntab : (mixed ? 0 : sizeof(double) * ntab);
+    if ((rc = allow_lds(s))) return fail(rc);
 
     // ---- region 0: contact bodies, load schedule, observables
     Region R0{};
@@ -2048,6 +2245,9 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.e0 = 0;
     R0.noff = sp.ok ? sp.noff : 0;
     R0.off0 = 0;
+    R0.nrow = sp.ok ? (int)sp.rows.size() : 0;
+    R0.row0 = 0;
+    R0.rx = sp.rx;
     R0.kg = sp.G;
     R0.tab0 = 0;
     R0.ntab = (int)(tabc.size() + tabp.size());
@@ -2821,7 +3021,7 @@ int batch_rebuild(bbpd_solver *s)
     s->P.nse = s->nse;
     s->lds_bytes = maxtab * (s->precision == BBPD_PRECISION_MIXED ? sizeof(float) : sizeof(double));
     s->version++;
-    return BBPD_OK;
+    return allow_lds(s);
 }
 
 bool same_bits(double a, double b) { return bits_equal(a, b); }
@@ -2885,6 +3085,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     uint4 *maskd;
     void *hsd, *s0d = nullptr, *scd = nullptr;
     int4 *stoffd;
+    int2 *srowd;
     float4 *sxid;
     float *svcfd, *tabfd = nullptr;
     double *tabd, *partd;
@@ -2903,6 +3104,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if (sh->s0 && (rc = dalloc((char **)&s0d, (size_t)nE * ew, s->owned))) return fail(rc);
     if (sh->sc && (rc = dalloc((char **)&scd, (size_t)nE * ew, s->owned))) return fail(rc);
     if ((rc = dalloc(&stoffd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
+    if ((rc = dalloc(&srowd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
     if ((rc = dalloc(&sxid, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
     if ((rc = dalloc(&svcfd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
     if ((rc = dalloc(&tabd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
@@ -2991,6 +3193,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     P.tick_done = s->d_tick_done;
     P.stencil = 1;
     P.soff = stoffd;
+    P.srow = srowd;
     P.sxi = sxid;
     P.svcf = svcfd;
     P.smask = maskd;
@@ -3190,6 +3393,9 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.e0 = e0;
     RG.noff = noff;
     RG.off0 = rid * ST_MAX;
+    RG.nrow = (int)sp.rows.size();
+    RG.row0 = rid * ST_MAX;
+    RG.rx = sp.rx;
     RG.kg = sp.G;
     RG.tab0 = rid * RB_TAB;
     RG.ntab = (int)tab.size();
@@ -3246,6 +3452,8 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     if (pr->s0 && (rc = put_entries((void *)P.s0str, hs0))) return rc;
     if (pr->sc && (rc = put_entries((void *)P.scstr, hsc))) return rc;
     BBPD_HIP(hipMemcpy((int4 *)P.soff + RG.off0, sp.off.data(), sizeof(int4) * noff, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((int2 *)P.srow + RG.row0, sp.rows.data(), sizeof(int2) * sp.rows.size(),
+                       hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy((float4 *)P.sxi + RG.off0, hxi.data(), sizeof(float4) * noff, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy((float *)P.svcf + RG.off0, hv.data(), sizeof(float) * noff, hipMemcpyHostToDevice));
     if (!tab.empty()) {
