@@ -65,6 +65,7 @@ constexpr int ST_ROW = 96;     // sites of a staged stencil row: 64 + 2 * ST_RMA
 constexpr unsigned char CONTACT_BIT = 1u << 3;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x8 __attribute__((ext_vector_type(8)));
 
 // how a per-bond quantity reaches the kernel
 enum ParamMode { PM_SCALAR = 0, PM_TABLE = 1, PM_STREAM = 2, PM_INLINE = 3 };
@@ -78,6 +79,7 @@ struct Region {
     long long e0;                  // stencil layout: first entry of the region
     int noff, off0;                // stencil layout: offsets off0 .. off0 + noff - 1
     int nrow, row0, rx;            // stencil layout: rows row0 .. row0 + nrow - 1, largest |dx|
+    int sst0;                      // stencil layout: first per-offset record
     int kg, tab0, ntab, ts0, tsc;  // stencil layout: key tables (ts0 / tsc relative to tab0)
     double s0, s1, sc;             // scalar law parameters
     int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
@@ -112,6 +114,14 @@ struct __align__(32) NodeRec {
     double x, y, z; // position: deformed coordinates (FP64 form) or displacement (mixed form)
     uint64_t q;     // packed lattice coordinates kx | ky << 18 | kz << 36, bond count << 56
 };
+
+// stencil layout, per offset (one 32-byte scalar load)
+struct __align__(32) StOff {
+    float x, y, z, L0; // mixed form: xi = h * d rounded to FP32, and |xi|
+    float vcf;         // mixed form: volume correction factor of |h * d|
+    int dx, dy, dz;    // the lattice offset d
+};
+constexpr int ST_PAD = 2; // per-offset entries past the last (copies of it): loads run two offsets ahead
 
 struct Params {
     // nodes
@@ -157,8 +167,7 @@ struct Params {
     // stencil layout (full lattice box, rows in lattice-offset order): neighbours are implicit
     int stencil;
     const int4 *soff;     // per offset: slot delta, dx, dy, dz
-    const float4 *sxi;    // per offset (mixed form): xi = h * d in FP32, |xi|
-    const float *svcf;    // per offset (mixed form): volume correction factor of |h * d|
+    const StOff *sst;     // per offset, ST_PAD entries past each region's last
     const int2 *srow;     // per stencil row (offsets sharing dy, dz): offset count, slot delta at dx = 0
     const uint4 *smask;   // per slot: bit o set when the node has the bond at offset o
     void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
@@ -515,11 +524,7 @@ template <typename T>
 struct StEnt {
     T h;      // stored history (sign = flag)
     T p0, p1; // streamed s0, sc
-    // per-offset data (the same for every lane): mixed form xi, vcf and dx; FP64 form the offset
-    float4 xi;
-    float vcf;
-    int dx;
-    int4 so;
+    StOff st; // per-offset data (the same for every lane)
 };
 
 // one stencil row of neighbour records in flight: sites 0 .. 63 (a) and 64 .. 63 + 2 rx (b)
@@ -534,8 +539,10 @@ using sconst_ptr = const __attribute__((address_space(4))) T *;
 template <typename T>
 __device__ __forceinline__ T sload(const T *p, int i)
 {
-    static_assert(sizeof(T) == 4 || sizeof(T) == 16, "32-bit or 128-bit elements");
-    if constexpr (sizeof(T) == 16)
+    static_assert(sizeof(T) == 4 || sizeof(T) == 16 || sizeof(T) == 32, "32-, 128- or 256-bit elements");
+    if constexpr (sizeof(T) == 32)
+        return __builtin_bit_cast(T, ((sconst_ptr<u32x8>)p)[i]);
+    else if constexpr (sizeof(T) == 16)
         return __builtin_bit_cast(T, ((sconst_ptr<u32x4>)p)[i]);
     else
         return __builtin_bit_cast(T, ((sconst_ptr<uint32_t>)p)[i]);
@@ -661,8 +668,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const int n0 = gb * BLOCK + wv * SLICE;
     const int rx = RG.rx, nrow = RG.nrow;
     const int *rows = reinterpret_cast<const int *>(P.srow + RG.row0); // (count, delta) per row
-    const int4 *offs = P.soff + off0;
-    const int *dls = reinterpret_cast<const int *>(offs); // offset o: dls[4 * o] = delta, dls[4 * o + 1] = dx
     auto fetch = [&](RowRecs &R, int r) {
         const uint32_t sb = (uint32_t)(n0 + sload(rows, 2 * r + 1) - rx);
         R.a = bload_rec(rp, (sb + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
@@ -684,25 +689,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         if (lane < 2 * rx) put(w, SLICE + lane, R.b);
     };
     double f0 = 0, f1 = 0, f2 = 0;
+    const StOff *sst = P.sst + RG.sst0;
+    // offset o's loads; o runs to noff: the entry arrays end at least two slices' worth past the
+    // last entry and the per-offset table ST_PAD records past the last offset
     auto load = [&](StEnt<T> &d, int o) {
-        o = min(o, noff - 1);
         const uint32_t so = (uint32_t)o * (SLICE * TW);
         d.h = bload_nt<T>(rh, lo, so);
         if (GEN && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lo, so);
         if (GEN && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lo, so);
-        if constexpr (MIXED) {
-            d.xi = sload(P.sxi + off0, o);
-            d.vcf = sload(P.svcf + off0, o);
-            d.dx = sload(dls, 4 * o + 1);
-        } else {
-            d.so = sload(offs, o);
-            d.dx = d.so.y;
-        }
+        d.st = sload(sst, o);
     };
     auto compute = [&](const StEnt<T> &d, int o, bool bit, int buf) {
         if (!bit) return;
         const uint32_t so = (uint32_t)o * (SLICE * TW);
-        const double *nb = &s_row[buf][wv][0] + (lane + rx + d.dx);
+        const double *nb = &s_row[buf][wv][0] + (lane + rx + d.st.dx);
         const double nbx = nb[0], nby = nb[ST_ROW], nbz = nb[2 * ST_ROW];
         const int key = gme * kg + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
         constexpr uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
@@ -735,9 +735,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
                 bstore<T>(hn, rh, lo, so);
         };
         if constexpr (!MIXED) {
-            const int4 sof = d.so;
-            const double Xm0 = P.hx * (double)(kx + sof.y), Xm1 = P.hy * (double)(ky + sof.z),
-                         Xm2 = P.hz * (double)(kz + sof.w);
+            const double Xm0 = P.hx * (double)(kx + d.st.dx), Xm1 = P.hy * (double)(ky + d.st.dy),
+                         Xm2 = P.hz * (double)(kz + d.st.dz);
             const double c = tab[key];
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
             const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
@@ -766,8 +765,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             f1 += gy;
             f2 += gz;
         } else {
-            const float4 xi = d.xi;
-            const float kap = tab[key] * d.vcf;
+            const float4 xi = make_float4(d.st.x, d.st.y, d.st.z, d.st.L0);
+            const float kap = tab[key] * d.st.vcf;
             const float et0 = (float)(nbx - qn0), et1 = (float)(nby - qn1), et2 = (float)(nbz - qn2);
             const float y0 = xi.x + et0, y1 = xi.y + et1, y2 = xi.z + et2;
             const float L0 = xi.w;
@@ -1607,6 +1606,28 @@ struct StencilPlan {
     int s0mode = PM_SCALAR, scmode = PM_SCALAR;
 };
 
+// Per-offset records of the stencil layout: xi = h * d rounded to FP32 per component (an offset and
+// its negation give exactly negated vectors), |xi| from those, the volume correction factor of
+// |h * d| (calculatevolumecorrectionfactors.m:20-32), and d; then ST_PAD copies of the last.
+std::vector<StOff> offset_records(const std::vector<int4> &off, const double *h, double hmr, double hpr,
+                                  double horizon, double two_r)
+{
+    std::vector<StOff> r(off.size());
+    for (size_t o = 0; o < off.size(); o++) {
+        const double e0 = h[0] * off[o].y, e1 = h[1] * off[o].z, e2 = h[2] * off[o].w;
+        const float x0 = (float)e0, x1 = (float)e1, x2 = (float)e2;
+        const double L0 = std::sqrt((e0 * e0 + e1 * e1) + e2 * e2);
+        double vcf;
+        if (L0 <= hmr) vcf = 1;
+        else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
+        else vcf = 0;
+        r[o] = StOff{x0, x1, x2, std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2))), (float)vcf,
+                     off[o].y, off[o].z, off[o].w};
+    }
+    for (int k = 0; k < ST_PAD && !r.empty(); k++) r.push_back(r.back());
+    return r;
+}
+
 bool key_table(const double *a, long long nB, const std::vector<int> &key, int nkey, std::vector<double> &tab)
 {
     tab.assign(nkey, 0.0);
@@ -2037,22 +2058,21 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     uint4 *maskd = nullptr;
     int4 *stoffd = nullptr;
     int2 *srowd = nullptr;
-    float4 *sxid = nullptr;
-    float *svcfd = nullptr, *tabfd = nullptr;
+    StOff *sstd = nullptr;
+    float *tabfd = nullptr;
     if (sp.ok) {
-        if ((rc = dalloc((char **)&hsd, (size_t)nE * (mixed ? 4 : 8), s->owned))) return fail(rc);
+        if ((rc = dalloc((char **)&hsd, (size_t)(nE + 2 * SLICE) * (mixed ? 4 : 8), s->owned))) return fail(rc);
         if ((rc = dalloc(&maskd, nSlots, s->owned))) return fail(rc);
         if ((rc = dalloc(&stoffd, sp.noff, s->owned))) return fail(rc);
         if ((rc = dalloc(&srowd, sp.rows.size(), s->owned))) return fail(rc);
-        if ((rc = dalloc(&sxid, sp.noff, s->owned))) return fail(rc);
-        if ((rc = dalloc(&svcfd, sp.noff, s->owned))) return fail(rc);
+        if ((rc = dalloc(&sstd, sp.noff + ST_PAD, s->owned))) return fail(rc);
     }
     double *cstrd = nullptr, *tabd = nullptr;
     void *s0d = nullptr, *scd = nullptr;
     const size_t ew = mixed ? 4 : 8;
     if (!h_c64.empty() && (rc = dalloc(&cstrd, nE, s->owned))) return fail(rc);
-    if (s0mode == PM_STREAM && (rc = dalloc((char **)&s0d, nE * ew, s->owned))) return fail(rc);
-    if (scmode == PM_STREAM && (rc = dalloc((char **)&scd, nE * ew, s->owned))) return fail(rc);
+    if (s0mode == PM_STREAM && (rc = dalloc((char **)&s0d, (nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
+    if (scmode == PM_STREAM && (rc = dalloc((char **)&scd, (nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     const size_t ntab = tabc.size() + tabp.size();
     if ((rc = dalloc(&tabd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
     if (sp.ok && mixed && (rc = dalloc(&tabfd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
@@ -2139,21 +2159,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         BBPD_HIP(hipMemcpy(srowd, sp.rows.data(), sizeof(int2) * sp.rows.size(), hipMemcpyHostToDevice));
         // mixed form per offset: xi = h * d rounded to FP32 (so the offset and its negation give
         // exactly negated vectors), |xi|, and the volume correction factor of |h * d|
-        std::vector<float4> hx(sp.noff);
-        std::vector<float> hv(sp.noff);
-        for (int o = 0; o < sp.noff; o++) {
-            const double e0 = hlat[0] * sp.off[o].y, e1 = hlat[1] * sp.off[o].z, e2 = hlat[2] * sp.off[o].w;
-            const float x0 = (float)e0, x1 = (float)e1, x2 = (float)e2;
-            hx[o] = make_float4(x0, x1, x2, std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2))));
-            const double L0 = std::sqrt((e0 * e0 + e1 * e1) + e2 * e2);
-            double vcf;
-            if (L0 <= hmr) vcf = 1;
-            else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
-            else vcf = 0;
-            hv[o] = (float)vcf;
-        }
-        BBPD_HIP(hipMemcpy(sxid, hx.data(), sizeof(float4) * sp.noff, hipMemcpyHostToDevice));
-        BBPD_HIP(hipMemcpy(svcfd, hv.data(), sizeof(float) * sp.noff, hipMemcpyHostToDevice));
+        const std::vector<StOff> hst = offset_records(sp.off, hlat, hmr, hpr, horizon, two_r);
+        BBPD_HIP(hipMemcpy(sstd, hst.data(), sizeof(StOff) * hst.size(), hipMemcpyHostToDevice));
         if (mixed) {
             // c * V, then the parameter tables, in FP32
             std::vector<float> tf(ntab, 0.0f);
@@ -2227,8 +2234,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.stencil = sp.ok ? 1 : 0;
     P.soff = stoffd;
     P.srow = srowd;
-    P.sxi = sxid;
-    P.svcf = svcfd;
+    P.sst = sstd;
     P.smask = maskd;
     P.hs = hsd;
     P.tabf = tabfd;
@@ -2248,6 +2254,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.nrow = sp.ok ? (int)sp.rows.size() : 0;
     R0.row0 = 0;
     R0.rx = sp.rx;
+    R0.sst0 = 0;
     R0.kg = sp.G;
     R0.tab0 = 0;
     R0.ntab = (int)(tabc.size() + tabp.size());
@@ -3086,8 +3093,8 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     void *hsd, *s0d = nullptr, *scd = nullptr;
     int4 *stoffd;
     int2 *srowd;
-    float4 *sxid;
-    float *svcfd, *tabfd = nullptr;
+    StOff *sstd;
+    float *tabfd = nullptr;
     double *tabd, *partd;
     if ((rc = dalloc(&Xd, 3 * nS, s->owned))) return fail(rc);
     for (int q = 0; q < 2; q++)
@@ -3100,13 +3107,12 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if (sh->integrator == BBPD_INTEGRATOR_ADR && (rc = dalloc(&massd, nS, s->owned))) return fail(rc);
     if ((rc = dalloc(&consd, nS, s->owned))) return fail(rc);
     if ((rc = dalloc(&maskd, nS, s->owned))) return fail(rc);
-    if ((rc = dalloc((char **)&hsd, (size_t)nE * ew, s->owned))) return fail(rc);
-    if (sh->s0 && (rc = dalloc((char **)&s0d, (size_t)nE * ew, s->owned))) return fail(rc);
-    if (sh->sc && (rc = dalloc((char **)&scd, (size_t)nE * ew, s->owned))) return fail(rc);
+    if ((rc = dalloc((char **)&hsd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
+    if (sh->s0 && (rc = dalloc((char **)&s0d, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
+    if (sh->sc && (rc = dalloc((char **)&scd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if ((rc = dalloc(&stoffd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
     if ((rc = dalloc(&srowd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
-    if ((rc = dalloc(&sxid, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
-    if ((rc = dalloc(&svcfd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
+    if ((rc = dalloc(&sstd, (size_t)s->maxRegions * (ST_MAX + ST_PAD), s->owned))) return fail(rc);
     if ((rc = dalloc(&tabd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
     if (mixed && (rc = dalloc(&tabfd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
     if ((rc = dalloc(&partd, 2 * (size_t)nblk + 2, s->owned))) return fail(rc);
@@ -3194,8 +3200,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     P.stencil = 1;
     P.soff = stoffd;
     P.srow = srowd;
-    P.sxi = sxid;
-    P.svcf = svcfd;
+    P.sst = sstd;
     P.smask = maskd;
     P.hs = hsd;
     s->device_bytes = nS * (2 * sizeof(NodeRec) + 8 * (3 * (mixed ? 3 : 4) + 2 + (Fpd ? 3 : 0)) + 1 + 16) +
@@ -3361,20 +3366,9 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
         for (int n = 0; n < nN; n++)
             if (in_family[n]) hc[sp.slot[n]] |= CONTACT_BIT;
     // per-offset data and key tables
-    std::vector<float4> hxi(noff);
-    std::vector<float> hv(noff);
     const double hmr = sh.horizon - sh.RADIJ, hpr = sh.horizon + sh.RADIJ, two_r = 2 * sh.RADIJ;
-    for (int o = 0; o < noff; o++) {
-        const double a0 = s->DX * sp.off[o].y, a1 = s->DX * sp.off[o].z, a2 = s->DX * sp.off[o].w;
-        const float x0 = (float)a0, x1 = (float)a1, x2 = (float)a2;
-        hxi[o] = make_float4(x0, x1, x2, std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2))));
-        const double L0 = std::sqrt((a0 * a0 + a1 * a1) + a2 * a2);
-        double vcf;
-        if (L0 <= hmr) vcf = 1;
-        else if ((L0 > hmr) && (L0 <= sh.horizon)) vcf = (hpr - L0) / two_r;
-        else vcf = 0;
-        hv[o] = (float)vcf;
-    }
+    const double hdx[3] = {s->DX, s->DX, s->DX};
+    const std::vector<StOff> hst = offset_records(sp.off, hdx, hmr, hpr, sh.horizon, two_r);
     std::vector<double> tab(sp.tc);
     tab.insert(tab.end(), sp.ts0.begin(), sp.ts0.end());
     tab.insert(tab.end(), sp.tsc.begin(), sp.tsc.end());
@@ -3396,6 +3390,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.nrow = (int)sp.rows.size();
     RG.row0 = rid * ST_MAX;
     RG.rx = sp.rx;
+    RG.sst0 = rid * (ST_MAX + ST_PAD);
     RG.kg = sp.G;
     RG.tab0 = rid * RB_TAB;
     RG.ntab = (int)tab.size();
@@ -3454,8 +3449,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     BBPD_HIP(hipMemcpy((int4 *)P.soff + RG.off0, sp.off.data(), sizeof(int4) * noff, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy((int2 *)P.srow + RG.row0, sp.rows.data(), sizeof(int2) * sp.rows.size(),
                        hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy((float4 *)P.sxi + RG.off0, hxi.data(), sizeof(float4) * noff, hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy((float *)P.svcf + RG.off0, hv.data(), sizeof(float) * noff, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((StOff *)P.sst + RG.sst0, hst.data(), sizeof(StOff) * hst.size(), hipMemcpyHostToDevice));
     if (!tab.empty()) {
         BBPD_HIP(hipMemcpy((double *)P.tab + RG.tab0, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
         if (P.tabf)
