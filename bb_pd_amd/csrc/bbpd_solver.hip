@@ -69,6 +69,8 @@ typedef unsigned int u32x8 __attribute__((ext_vector_type(8)));
 
 // how a per-bond quantity reaches the kernel
 enum ParamMode { PM_SCALAR = 0, PM_TABLE = 1, PM_STREAM = 2, PM_INLINE = 3 };
+// stencil bond kernel variants by where s0 and sc come from
+enum ParamVariant { PV_SCALAR = 0, PV_TABLE = 1, PV_GENERAL = 2 };
 
 // A region is one problem instance: a block-aligned range of slots with its own stencil, tables,
 // law scalars, contact bodies, load schedule, history and step counter.  A single problem is one
@@ -168,7 +170,8 @@ struct Params {
     int stencil;
     const int4 *soff;     // per offset: slot delta, dx, dy, dz
     const StOff *sst;     // per offset, ST_PAD entries past each region's last
-    const int2 *srow;     // per stencil row (offsets sharing dy, dz): offset count, slot delta at dx = 0
+    const int2 *srow;     // per stencil row (offsets sharing dy, dz): offset count, slot delta at dx = 0;
+                          // ST_PAD entries past each region's last
     const uint4 *smask;   // per slot: bit o set when the node has the bond at offset o
     void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
     const float *tabf;    // mixed-form key tables (c * V, then parameter tables)
@@ -609,9 +612,10 @@ __device__ __forceinline__ bool law_active(T s, T s0, T sc, T s1, bool flag)
 
 // Launch variants, picked per solver by an autotuner: OCC is the minimum waves per SIMD the compiler
 // must allow (1 = its own choice), which bounds the registers per wave, trading instruction-level
-// for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.  GEN takes
-// s0 and sc from key tables or per-entry streams; without it they are the region's scalars.
-template <bool MIXED, int LAW, int OCC, bool REMAP, bool GEN>
+// for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.  PV says
+// where s0 and sc come from: PV_SCALAR the region's scalars, PV_TABLE both from key tables,
+// PV_GENERAL each from its scalar, key table or per-entry stream as the solver's modes say.
+template <bool MIXED, int LAW, int OCC, bool REMAP, int PV>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) void k_bond_st(Params P)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
@@ -653,8 +657,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const long long eslice = RG.e0 + ((long long)((gb * BLOCK - base) / SLICE) + wv) * noff * SLICE;
     const uint32_t sbytes = (uint32_t)noff * SLICE * TW;
     const brsrc rh = make_rsrc((const T *)P.hs + eslice, sbytes);
-    const brsrc r0 = GEN && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
-    const brsrc r1 = GEN && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
+    const brsrc r0 = PV == PV_GENERAL && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
+    const brsrc r1 = PV == PV_GENERAL && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
     const uint32_t lo = (threadIdx.x % SLICE) * TW;
     // Neighbour records, staged per stencil row (the offsets that share dy and dz, consecutive in
     // offset order).  For the slice starting at slot n0, row r covers the slots
@@ -668,8 +672,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const int n0 = gb * BLOCK + wv * SLICE;
     const int rx = RG.rx, nrow = RG.nrow;
     const int *rows = reinterpret_cast<const int *>(P.srow + RG.row0); // (count, delta) per row
-    auto fetch = [&](RowRecs &R, int r) {
-        const uint32_t sb = (uint32_t)(n0 + sload(rows, 2 * r + 1) - rx);
+    auto fetch = [&](RowRecs &R, int dl) {
+        const uint32_t sb = (uint32_t)(n0 + dl - rx);
         R.a = bload_rec(rp, (sb + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
         if (lane < 2 * rx) R.b = bload_rec(rp, (sb + SLICE + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
     };
@@ -695,8 +699,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     auto load = [&](StEnt<T> &d, int o) {
         const uint32_t so = (uint32_t)o * (SLICE * TW);
         d.h = bload_nt<T>(rh, lo, so);
-        if (GEN && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lo, so);
-        if (GEN && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lo, so);
+        if (PV == PV_GENERAL && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lo, so);
+        if (PV == PV_GENERAL && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lo, so);
         d.st = sload(sst, o);
     };
     auto compute = [&](const StEnt<T> &d, int o, bool bit, int buf) {
@@ -719,7 +723,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             flag = (long long)hb < 0;
         }
         T s0 = rs0, sc = rsc;
-        if (GEN) {
+        if (PV == PV_TABLE) {
+            s0 = tab[ts0 + key];
+            sc = tab[tsc + key];
+        } else if (PV == PV_GENERAL) {
             s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : rs0);
             sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : rsc);
         }
@@ -785,17 +792,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     };
     if (noff > 0) {
         RowRecs R;
-        fetch(R, 0);
+        fetch(R, sload(rows, 1));
         stage(R, 0);
         int r = 0, rend = sload(rows, 0);
-        if (nrow > 1) fetch(R, 1);
+        if (nrow > 1) fetch(R, sload(rows, 3));
+        // the next row's offset count and the slot delta of the row after it, read a row ahead
+        int cnext = sload(rows, 2), dnext = sload(rows, 5);
         // entering offset o: at a row's end, stage the next row and start loading the one after
         auto next_row = [&](int o) {
             if (o == rend) {
                 r++;
                 stage(R, r & 1);
-                rend += sload(rows, 2 * r);
-                if (r + 1 < nrow) fetch(R, r + 1);
+                rend += cnext;
+                if (r + 1 < nrow) fetch(R, dnext);
+                cnext = sload(rows, 2 * (r + 1));
+                dnext = sload(rows, 2 * (r + 2) + 1);
             }
         };
         StEnt<T> A, B;
@@ -1385,25 +1396,35 @@ KernelFn pick_law(int law, bool lat)
 // launch variants of the stencil bond kernel: (minimum waves per SIMD, XCD block remap)
 constexpr int N_OCC = 4;
 
-template <bool MIXED, int LAW, bool GEN>
+template <bool MIXED, int LAW, int PV>
 KernelFn pick_occ(int v)
 {
     switch (v) {
-    case 1: return k_bond_st<MIXED, LAW, 1, false, GEN>;
-    case 2: return k_bond_st<MIXED, LAW, 5, true, GEN>;
-    case 3: return k_bond_st<MIXED, LAW, 8, true, GEN>;
-    default: return k_bond_st<MIXED, LAW, 1, true, GEN>;
+    case 1: return k_bond_st<MIXED, LAW, 1, false, PV>;
+    case 2: return k_bond_st<MIXED, LAW, 5, true, PV>;
+    case 3: return k_bond_st<MIXED, LAW, 8, true, PV>;
+    default: return k_bond_st<MIXED, LAW, 1, true, PV>;
     }
 }
 
-template <bool MIXED, bool GEN>
+template <bool MIXED, int PV>
 KernelFn pick_law_st(int law, int v)
 {
     switch (law) {
-    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR, GEN>(v);
-    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR, GEN>(v);
-    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR, GEN>(v);
-    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL, GEN>(v);
+    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR, PV>(v);
+    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR, PV>(v);
+    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR, PV>(v);
+    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL, PV>(v);
+    }
+}
+
+template <bool MIXED>
+KernelFn pick_pv_st(int pv, int law, int v)
+{
+    switch (pv) {
+    case PV_SCALAR: return pick_law_st<MIXED, PV_SCALAR>(law, v);
+    case PV_TABLE: return pick_law_st<MIXED, PV_TABLE>(law, v);
+    default: return pick_law_st<MIXED, PV_GENERAL>(law, v);
     }
 }
 
@@ -1412,10 +1433,10 @@ KernelFn pick_bond_kernel(const bbpd_solver *s, int variant = -1)
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     const int v = variant >= 0 ? variant : s->occ;
     if (s->stencil) {
-        // s0 and sc as the region's scalars, or from key tables / per-entry streams
-        const bool gen = s->P.s0mode != PM_SCALAR || s->P.scmode != PM_SCALAR;
-        if (mixed) return gen ? pick_law_st<true, true>(s->law, v) : pick_law_st<true, false>(s->law, v);
-        return gen ? pick_law_st<false, true>(s->law, v) : pick_law_st<false, false>(s->law, v);
+        const int m0 = s->P.s0mode, mc = s->P.scmode;
+        const int pv = (m0 == PM_SCALAR && mc == PM_SCALAR) ? PV_SCALAR
+                       : (m0 == PM_TABLE && mc == PM_TABLE) ? PV_TABLE : PV_GENERAL;
+        return mixed ? pick_pv_st<true>(pv, s->law, v) : pick_pv_st<false>(pv, s->law, v);
     }
     return mixed ? pick_law<true>(s->law, s->lattice) : pick_law<false>(s->law, s->lattice);
 }
@@ -1593,7 +1614,9 @@ struct StencilPlan {
     std::string why;              // reason the layout does not apply
     int noff = 0;                 // offsets, padded to even
     std::vector<int4> off;        // slot delta, dx, dy, dz
-    std::vector<int2> rows;       // offset count and slot delta of dx = 0, per run of equal (dy, dz)
+    std::vector<int2> rows;       // offset count and slot delta of dx = 0, per run of equal (dy, dz);
+                                  // nrow rows, then ST_PAD padding entries
+    int nrow = 0;
     int rx = 0;                   // largest |dx|
     std::vector<int> slot;        // node -> slot
     std::vector<int> oi, oj;      // per bond: offset index at the i end and at the j end
@@ -1733,6 +1756,8 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
         o = e;
     }
     for (const int4 &q : sp.off) sp.rx = std::max(sp.rx, std::abs(q.y));
+    sp.nrow = (int)sp.rows.size();
+    for (int k = 0; k < ST_PAD; k++) sp.rows.push_back(make_int2(0, 0)); // read ahead, never a row
     // rows in ascending offset order, each offset at most once per node
     sp.oi.assign(nB, 0);
     sp.oj.assign(nB, 0);
@@ -1788,6 +1813,15 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
     if (pr->sc) sp.scmode = pair_table(pr->sc, sp.tsc) ? PM_TABLE : PM_STREAM;
     if (sp.s0mode != PM_TABLE) sp.ts0.clear();
     if (sp.scmode != PM_TABLE) sp.tsc.clear();
+    // one of s0 / sc tabled and the other scalar: table the scalar too, so the bond kernel reads
+    // both from tables with no per-entry choice
+    if (sp.s0mode == PM_TABLE && sp.scmode == PM_SCALAR) {
+        sp.tsc.assign(sp.nkey, pr->sc_scalar);
+        sp.scmode = PM_TABLE;
+    } else if (sp.scmode == PM_TABLE && sp.s0mode == PM_SCALAR) {
+        sp.ts0.assign(sp.nkey, pr->s0_scalar);
+        sp.s0mode = PM_TABLE;
+    }
     sp.ok = true;
 }
 
@@ -2251,7 +2285,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.e0 = 0;
     R0.noff = sp.ok ? sp.noff : 0;
     R0.off0 = 0;
-    R0.nrow = sp.ok ? (int)sp.rows.size() : 0;
+    R0.nrow = sp.ok ? sp.nrow : 0;
     R0.row0 = 0;
     R0.rx = sp.rx;
     R0.sst0 = 0;
@@ -3111,7 +3145,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if (sh->s0 && (rc = dalloc((char **)&s0d, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if (sh->sc && (rc = dalloc((char **)&scd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if ((rc = dalloc(&stoffd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
-    if ((rc = dalloc(&srowd, (size_t)s->maxRegions * ST_MAX, s->owned))) return fail(rc);
+    if ((rc = dalloc(&srowd, (size_t)s->maxRegions * (ST_MAX + ST_PAD), s->owned))) return fail(rc);
     if ((rc = dalloc(&sstd, (size_t)s->maxRegions * (ST_MAX + ST_PAD), s->owned))) return fail(rc);
     if ((rc = dalloc(&tabd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
     if (mixed && (rc = dalloc(&tabfd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
@@ -3387,8 +3421,8 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.e0 = e0;
     RG.noff = noff;
     RG.off0 = rid * ST_MAX;
-    RG.nrow = (int)sp.rows.size();
-    RG.row0 = rid * ST_MAX;
+    RG.nrow = sp.nrow;
+    RG.row0 = rid * (ST_MAX + ST_PAD);
     RG.rx = sp.rx;
     RG.sst0 = rid * (ST_MAX + ST_PAD);
     RG.kg = sp.G;
