@@ -1882,16 +1882,15 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     const bool lattice = detect_lattice(pr, hlat, xmin, xmax, hq);
     s->lattice = lattice;
 
-    // ---- layout.  The FP32-mixed form takes the stencil layout where it applies: its bond kernel
-    // is bound by memory traffic, which the stencil layout cuts.  The plain FP64 form is bound by
-    // its FP64 arithmetic and runs a little faster on the record layout's node order, which it
-    // takes by default.  BBPD_LAYOUT=stencil / records overrides either.
+    // ---- layout.  Both forms take the stencil layout where it applies: it moves a fraction of the
+    // record layout's bytes per entry and loads each neighbour record once per stencil row.
+    // BBPD_LAYOUT=records asks for the record layout.
     StencilPlan sp;
     {
         const char *lay = std::getenv("BBPD_LAYOUT");
-        const bool want_records = lay ? std::strcmp(lay, "records") == 0 : !mixed;
+        const bool want_records = lay && std::strcmp(lay, "records") == 0;
         if (!lattice) sp.why = "not a lattice";
-        else if (want_records) sp.why = lay ? "record layout requested" : "FP64 default";
+        else if (want_records) sp.why = "record layout requested";
         else plan_stencil(pr, X, bi, bj, hlat, sp);
     }
     s->stencil = sp.ok;

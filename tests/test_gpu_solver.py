@@ -389,14 +389,17 @@ def test_stencil_layout_mixed_close_to_records(gpu):
 
 
 def test_layout_defaults_and_fallback(gpu):
-    """Defaults: stencil for FP32-mixed, records for FP64.  A bond list not sorted by node id takes
-    the record layout even when the stencil one is asked for."""
+    """Default: the stencil layout for both forms; BBPD_LAYOUT=records asks for the record layout.
+    A bond list not sorted by node id takes the record layout even when the stencil one is asked
+    for."""
     case = ec_case(_lib.LAW_BILINEAR)
     import os
     assert "BBPD_LAYOUT" not in os.environ
     with Solver(case, _lib.MIXED) as a, Solver(case, _lib.FP64) as b:
         assert a.layout() == "stencil"
-        assert b.layout() == "records (FP64 default)"
+        assert b.layout() == "stencil"
+    with _solver_with_layout(case, _lib.FP64, "records") as r:
+        assert r.layout() == "records (record layout requested)", r.layout()
     order = np.random.default_rng(1).permutation(case.nBonds)
     case.BONDLIST = np.asfortranarray(case.BONDLIST[order])
     for name in ("L0", "vcf", "c", "BONDTYPE", "fail", "s0"):
