@@ -58,8 +58,8 @@ constexpr uint64_t LAT_MASK = (1ull << LAT_BITS) - 1;
 constexpr int G_SHIFT = 56;    // node record: the node's bond count (nFAMILYMEMBERS) in the top byte
 constexpr int ST_MAX = 128;    // offsets of the stencil layout (bits of the per-node mask)
 constexpr int ST_GMAX = 45;    // distinct family sizes of the stencil layout (key table <= 45^2)
-constexpr int ST_RMAX = 15;    // largest |dx| of a stencil offset
-constexpr int ST_ROW = 96;     // sites of a staged stencil row: 64 + 2 * ST_RMAX, rounded up
+constexpr int ST_RMAX = 8;     // largest |dx| of a stencil offset
+constexpr int ST_ROW = 80;     // sites of a staged stencil row: 64 + 2 * ST_RMAX
 // node-flag bits 0-2: DOF d constrained; bit 3: member of a contact family (its DOFs are driven by
 // the rigid-body projection, so the ADR Rayleigh quotient leaves the node out)
 constexpr unsigned char CONTACT_BIT = 1u << 3;
@@ -594,20 +594,29 @@ __device__ __forceinline__ NodeRec bload_rec(brsrc r, uint32_t voff)
     return __builtin_bit_cast(NodeRec, w);
 }
 
-// True when the law can change the history H or the flag at stretch s.  Every value a law stores
-// lies in [0, 1] (calculatebsfexponentialMex.c:55-58 clamps; the others are maxima of values in
-// [0, 1]) and set-up and restore accept only such values; with BONDTYPE 0 (the stencil layout's
-// only type), a stretch at or below the law's lowest threshold then leaves H and the flag as they
-// are (calculatebondfailureMex.c:36-38, calculatebondsofteningfactor.m:42-48,
-// calculatebsftrilinear.m:80-112, calculatebsfexponentialMex.c:35-58).  A bilinear bond, once
-// flagged, is evaluated every step: in compression its candidate is positive.
+// The law's lowest threshold.  Every value a law stores lies in [0, 1]
+// (calculatebsfexponentialMex.c:55-58 clamps; the others are maxima of values in [0, 1]) and set-up
+// and restore accept only such values; with BONDTYPE 0 (the stencil layout's only type), a stretch
+// at or below this threshold then leaves H and the flag as they are (calculatebondfailureMex.c:36-38,
+// calculatebondsofteningfactor.m:42-48, calculatebsftrilinear.m:80-112,
+// calculatebsfexponentialMex.c:35-58).
 template <int LAW, typename T>
-__device__ __forceinline__ bool law_active(T s, T s0, T sc, T s1, bool flag)
+__device__ __forceinline__ T law_threshold(T s0, T sc, T s1)
 {
-    if (LAW == BBPD_LAW_LINEAR) return s > sc;
-    if (LAW == BBPD_LAW_BILINEAR) return flag || s > s0;
-    if (LAW == BBPD_LAW_TRILINEAR) return s > fmin(fmin(s0, s1), sc);
-    return s > fmin(s0, sc);
+    if (LAW == BBPD_LAW_LINEAR) return sc;
+    if (LAW == BBPD_LAW_BILINEAR) return s0;
+    if (LAW == BBPD_LAW_TRILINEAR) return fmin(fmin(s0, s1), sc);
+    return fmin(s0, sc);
+}
+
+// True when the law can change the history H or the flag at stretch s; `thr` = law_threshold(s0,
+// sc, s1).  A bilinear bond, once flagged, is evaluated every step: in compression its candidate is
+// positive.
+template <int LAW, typename T>
+__device__ __forceinline__ bool law_active(T s, T thr, bool flag)
+{
+    if (LAW == BBPD_LAW_BILINEAR) return flag || s > thr;
+    return s > thr;
 }
 
 // Launch variants, picked per solver by an autotuner: OCC is the minimum waves per SIMD the compiler
@@ -621,8 +630,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     using T = typename std::conditional<MIXED, float, double>::type;
     constexpr uint32_t TW = sizeof(T);
     __shared__ double sh[BLOCK];
-    // per wavefront, two row buffers: x, y, z and the family-size index (low word) of ST_ROW sites
-    __shared__ double s_row[2][BLOCK / SLICE][4 * ST_ROW];
+    // per wavefront, a row buffer: x, y, z and the key word (low word) of ST_ROW sites.  One buffer
+    // suffices: a wavefront's LDS accesses complete in program order, so staging the next row
+    // (after the last read of this one) cannot overtake a read.
+    __shared__ double s_row[BLOCK / SLICE][4 * ST_ROW];
     extern __shared__ double tabs[]; // key tables (FP64, or FP32 in the mixed form)
     const int gb = launch_block(P, REMAP);
     if (gb < 0) return;
@@ -631,24 +642,35 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const int base = RG.base, noff = RG.noff, off0 = RG.off0, kg = RG.kg, tab0 = RG.tab0, ntab = RG.ntab;
     const int ts0 = RG.ts0, tsc = RG.tsc;
     const bool real = n - base < RG.nN;
-    const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc;
+    const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = law_threshold<LAW, T>(rs0, rsc, rs1);
     const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
     const int cur = (int)(*P.tick & 1);
     const NodeRec *__restrict__ pos = P.pos[cur];
-    if constexpr (MIXED) {
-        float *tf = reinterpret_cast<float *>(tabs);
-        for (int i = threadIdx.x; i < ntab; i += BLOCK) tf[i] = P.tabf[tab0 + i];
+    // key tables in LDS.  PV_TABLE keeps one (c, law threshold) record per key, and reads s0 and sc
+    // from the global tables when the law runs; otherwise the tables follow one another (c, then s0
+    // and sc where tabled).
+    const T *tsrc = MIXED ? reinterpret_cast<const T *>(P.tabf) : reinterpret_cast<const T *>(P.tab);
+    T *tdst = reinterpret_cast<T *>(tabs);
+    if (PV == PV_TABLE) {
+        for (int i = threadIdx.x; i < kg * kg; i += BLOCK) {
+            tdst[2 * i] = tsrc[tab0 + i];
+            tdst[2 * i + 1] = law_threshold<LAW, T>(tsrc[tab0 + ts0 + i], tsrc[tab0 + tsc + i], (T)RG.s1);
+        }
     } else {
-        for (int i = threadIdx.x; i < ntab; i += BLOCK) tabs[i] = P.tab[tab0 + i];
+        for (int i = threadIdx.x; i < ntab; i += BLOCK) tdst[i] = tsrc[tab0 + i];
     }
     __syncthreads();
-    const T *tab = reinterpret_cast<const T *>(tabs);
+    const T *tab = tdst;
+    // a neighbour's staged key word: its family-size index, or (PV_SCALAR / PV_TABLE) that index's
+    // byte offset in a row of the key table, so one add gives the entry's table address
+    constexpr int KSCALE = PV == PV_GENERAL ? 1 : (PV == PV_TABLE ? 2 : 1) * (int)sizeof(T);
 
     const NodeRec me = pos[n];
     const double qn0 = me.x, qn1 = me.y, qn2 = me.z;
     const int kx = (int)(me.q & LAT_MASK), ky = (int)((me.q >> LAT_BITS) & LAT_MASK),
               kz = (int)((me.q >> (2 * LAT_BITS)) & LAT_MASK);
     const int gme = (int)(me.q >> G_SHIFT); // index of the node's family size
+    const int krow = gme * kg * KSCALE;      // the node's row of the key table (index or bytes)
     // X == h * k exactly (checked at set-up)
     const double Xn0 = P.hx * (double)kx, Xn1 = P.hy * (double)ky, Xn2 = P.hz * (double)kz;
     const uint4 msk = P.smask[n];
@@ -659,7 +681,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const brsrc rh = make_rsrc((const T *)P.hs + eslice, sbytes);
     const brsrc r0 = PV == PV_GENERAL && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
     const brsrc r1 = PV == PV_GENERAL && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
-    const uint32_t lo = (threadIdx.x % SLICE) * TW;
+    const uint32_t lb = (threadIdx.x % SLICE) * TW;
     // Neighbour records, staged per stencil row (the offsets that share dy and dz, consecutive in
     // offset order).  For the slice starting at slot n0, row r covers the slots
     // n0 + delta_r - rx .. n0 + delta_r + 63 + rx; the wavefront loads them once into its LDS row
@@ -681,14 +703,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         w[site] = q.x;
         w[ST_ROW + site] = q.y;
         w[2 * ST_ROW + site] = q.z;
-        *reinterpret_cast<int *>(w + 3 * ST_ROW + site) = (int)(q.q >> G_SHIFT);
+        *reinterpret_cast<int *>(w + 3 * ST_ROW + site) = (int)(q.q >> G_SHIFT) * KSCALE;
         // the record's low lattice word is not used; keeping it live until here stops the register
         // allocator from reusing its register while the load is in flight (which would make the
         // wavefront wait for every outstanding load)
         __asm__ volatile("" ::"v"((uint32_t)q.q));
     };
-    auto stage = [&](const RowRecs &R, int buf) {
-        double *w = &s_row[buf][wv][0];
+    auto stage = [&](const RowRecs &R) {
+        double *w = &s_row[wv][0];
         put(w, lane, R.a);
         if (lane < 2 * rx) put(w, SLICE + lane, R.b);
     };
@@ -698,17 +720,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     // last entry and the per-offset table ST_PAD records past the last offset
     auto load = [&](StEnt<T> &d, int o) {
         const uint32_t so = (uint32_t)o * (SLICE * TW);
-        d.h = bload_nt<T>(rh, lo, so);
-        if (PV == PV_GENERAL && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lo, so);
-        if (PV == PV_GENERAL && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lo, so);
+        d.h = bload_nt<T>(rh, lb, so);
+        if (PV == PV_GENERAL && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lb, so);
+        if (PV == PV_GENERAL && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lb, so);
         d.st = sload(sst, o);
     };
-    auto compute = [&](const StEnt<T> &d, int o, bool bit, int buf) {
+    // One entry.  The FP64 form adds its force to the row sum here; the mixed form adds it to the
+    // FP32 sum `a` of a pair of entries (offsets o, o + 1), which the loop adds to the FP64 row sum.
+    auto compute = [&](const StEnt<T> &d, int o, bool bit, float3 &a) {
         if (!bit) return;
         const uint32_t so = (uint32_t)o * (SLICE * TW);
-        const double *nb = &s_row[buf][wv][0] + (lane + rx + d.st.dx);
+        const double *nb = &s_row[wv][0] + (lane + rx + d.st.dx);
         const double nbx = nb[0], nby = nb[ST_ROW], nbz = nb[2 * ST_ROW];
-        const int key = gme * kg + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
+        const int key = krow + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
+        // the entry's key-table values: c (c * V in the mixed form), s0, sc
+        const T *tk = PV == PV_GENERAL ? tab + key : reinterpret_cast<const T *>(reinterpret_cast<const char *>(tab) + key);
         constexpr uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
         // the history is |h| (+0 for -0), the flag its sign bit
         T H;
@@ -722,29 +748,34 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             H = __longlong_as_double((long long)(hb & 0x7FFFFFFFFFFFFFFFull));
             flag = (long long)hb < 0;
         }
-        T s0 = rs0, sc = rsc;
+        T s0 = rs0, sc = rsc, thr = rthr;
         if (PV == PV_TABLE) {
-            s0 = tab[ts0 + key];
-            sc = tab[tsc + key];
+            thr = tk[1];
         } else if (PV == PV_GENERAL) {
             s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : rs0);
             sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : rsc);
+            thr = law_threshold<LAW, T>(s0, sc, rs1);
         }
         // the law, and the store of a changed history (sign = flag)
         auto law = [&](T s) {
-            if (!law_active<LAW, T>(s, s0, sc, rs1, flag)) return;
+            if (!law_active<LAW, T>(s, thr, flag)) return;
+            if (PV == PV_TABLE) {
+                const int ki = key / KSCALE;
+                s0 = tsrc[tab0 + ts0 + ki];
+                sc = tsrc[tab0 + tsc + ki];
+            }
             uint32_t w = flag ? mark : 0u;
             apply_law<LAW, T>(s, s0, sc, rs1, lk, lalpha, ldenom, lsteel, lff, H, w);
             flag = (w & mark) != 0u;
             const T hn = flag ? -H : H;
             if (__builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, hn) !=
                 __builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, d.h))
-                bstore<T>(hn, rh, lo, so);
+                bstore<T>(hn, rh, lb, so);
         };
         if constexpr (!MIXED) {
             const double Xm0 = P.hx * (double)(kx + d.st.dx), Xm1 = P.hy * (double)(ky + d.st.dy),
                          Xm2 = P.hz * (double)(kz + d.st.dz);
-            const double c = tab[key];
+            const double c = tk[0];
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
             const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
@@ -773,27 +804,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             f2 += gz;
         } else {
             const float4 xi = make_float4(d.st.x, d.st.y, d.st.z, d.st.L0);
-            const float kap = tab[key] * d.st.vcf;
+            const float kap = tk[0] * d.st.vcf;
             const float et0 = (float)(nbx - qn0), et1 = (float)(nby - qn1), et2 = (float)(nbz - qn2);
             const float y0 = xi.x + et0, y1 = xi.y + et1, y2 = xi.z + et2;
             const float L0 = xi.w;
-            const float l = __builtin_amdgcn_sqrtf(fmaf(y0, y0, fmaf(y1, y1, y2 * y2)));
+            // |y|^2 carries 1e-30 (below every representable bond's square by many orders, so it
+            // changes no realistic length): 1/l stays finite for a bond of zero length, whose force
+            // p * y is then 0, the reference's NaN -> 0 (calculatebondforcesMex.c:38)
+            const float l2 = fmaf(y0, y0, fmaf(y1, y1, fmaf(y2, y2, 1e-30f)));
+            const float rl = __builtin_amdgcn_rsqf(l2);
+            const float l = l2 * rl;
             const float num =
                 fmaf(2.0f, fmaf(xi.x, et0, fmaf(xi.y, et1, xi.z * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
-            const float rl = __builtin_amdgcn_rcpf(l);
             const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
             law(s);
             const float fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0f : 1.0f;
-            const float p = mixed_force_factor(fail * kap * (1.0f - H) * s * rl);
-            f0 += (double)(p * y0);
-            f1 += (double)(p * y1);
-            f2 += (double)(p * y2);
+            const float p = fail * kap * (1.0f - H) * s * rl;
+            a.x = fmaf(p, y0, a.x);
+            a.y = fmaf(p, y1, a.y);
+            a.z = fmaf(p, y2, a.z);
         }
     };
     if (noff > 0) {
         RowRecs R;
         fetch(R, sload(rows, 1));
-        stage(R, 0);
+        stage(R);
         int r = 0, rend = sload(rows, 0);
         if (nrow > 1) fetch(R, sload(rows, 3));
         // the next row's offset count and the slot delta of the row after it, read a row ahead
@@ -802,7 +837,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         auto next_row = [&](int o) {
             if (o == rend) {
                 r++;
-                stage(R, r & 1);
+                stage(R);
                 rend += cnext;
                 if (r + 1 < nrow) fetch(R, dnext);
                 cnext = sload(rows, 2 * (r + 1));
@@ -816,12 +851,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             uint32_t mw = c == 0 ? msk.x : (c == 32 ? msk.y : (c == 64 ? msk.z : msk.w));
             const int oe = min(c + 32, noff);
             for (int o = c; o < oe; o += 2) {
+                float3 a = make_float3(0.0f, 0.0f, 0.0f);
                 load(B, o + 1);
                 next_row(o);
-                compute(A, o, (mw & 1u) != 0u, r & 1);
+                compute(A, o, (mw & 1u) != 0u, a);
                 load(A, o + 2);
                 next_row(o + 1);
-                compute(B, o + 1, (mw & 2u) != 0u, r & 1);
+                compute(B, o + 1, (mw & 2u) != 0u, a);
+                if constexpr (MIXED) {
+                    f0 += (double)a.x;
+                    f1 += (double)a.y;
+                    f2 += (double)a.z;
+                }
                 mw >>= 2;
             }
         }
@@ -1718,7 +1759,7 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
     for (int b = 0; b < nB; b++)
         for (int d = 0; d < 3; d++) R = std::max(R, std::abs(sp.k[3 * (size_t)bj[b] + d] - sp.k[3 * (size_t)bi[b] + d]));
     if (R > ST_RMAX) {
-        sp.why = "bond offsets longer than 15 lattice spacings";
+        sp.why = "bond offsets longer than " + std::to_string(ST_RMAX) + " lattice spacings";
         return;
     }
     const int Wd = 2 * R + 1;
@@ -2271,7 +2312,12 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.smask = maskd;
     P.hs = hsd;
     P.tabf = tabfd;
-    s->lds_bytes = sp.ok ? (mixed ? sizeof(float) : sizeof(double)) * ntab : (mixed ? 0 : sizeof(double) * ntab);
+    // key tables in LDS; with s0 and sc both tabled the stencil kernel keeps (c, law threshold) per key
+    {
+        const size_t w = mixed ? sizeof(float) : sizeof(double);
+        const bool both = s0mode == PM_TABLE && scmode == PM_TABLE;
+        s->lds_bytes = sp.ok ? w * (both ? (size_t)(2 * sp.G * sp.G) : ntab) : (mixed ? 0 : w * ntab);
+    }
     if ((rc = allow_lds(s))) return fail(rc);
 
     // ---- region 0: contact bodies, load schedule, observables
@@ -2835,13 +2881,13 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
         std::vector<double> hv;
         if ((rc = fetch_hs(s, hv))) return rc;
         for (int k = 0; k < s->nB; k++) {
-            const double cur = hv[s->entry_i[k]];
-            const double mag = H ? H[k] : std::fabs(cur);
-            bool sgn = std::signbit(cur);
+            const double was = hv[s->entry_i[k]];
+            const double mag = H ? H[k] : std::fabs(was);
+            bool sgn = std::signbit(was);
             if (!linear && flag) sgn = flag[k] == 1;
             if (linear && fail) sgn = fail[k] == 0;
-            const double v = sgn ? -mag : mag;
-            hv[s->entry_i[k]] = hv[s->entry_j[k]] = v;
+            const double hn = sgn ? -mag : mag;
+            hv[s->entry_i[k]] = hv[s->entry_j[k]] = hn;
         }
         if ((rc = put_hs(s, hv))) return rc;
     } else if (H || flag || fail) {

@@ -374,16 +374,23 @@ def test_stencil_layout_restore_and_adr(gpu):
 
 
 def test_stencil_layout_mixed_close_to_records(gpu):
-    """Mixed form: the two layouts round c*V*vcf and xi differently (FP32), nothing else."""
+    """Mixed form: the two layouts round differently (c*V*vcf and xi; the stencil kernel takes the
+    length through a reciprocal square root and adds pairs of entries in FP32 before the FP64 row
+    sum), nothing else: after 300 Euler-Cromer steps both stay as close to the FP64 run as to each
+    other."""
     case = ec_case(_lib.LAW_BILINEAR, steps=300)
     a = _solver_with_layout(case, _lib.MIXED, "stencil")
     b = _solver_with_layout(case, _lib.MIXED, "records")
-    with a, b:
+    c = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b, c:
         assert a.layout() == "stencil"
-        a.step(300)
-        b.step(300)
-        ua, ub = a.nodal()[0], b.nodal()[0]
-        assert np.abs(ua - ub).max() <= 1e-5 * np.abs(ub).max()
+        for s in (a, b, c):
+            s.step(300)
+        ua, ub, uc = a.nodal()[0], b.nodal()[0], c.nodal()[0]
+        scale = np.abs(uc).max()
+        d_ab, d_ac, d_bc = (np.abs(x - y).max() / scale for x, y in ((ua, ub), (ua, uc), (ub, uc)))
+        print(f"mixed stencil vs records {d_ab:.2e}, stencil vs FP64 {d_ac:.2e}, records vs FP64 {d_bc:.2e}")
+        assert d_ab <= 1e-4 and d_ac <= 1e-4 and d_bc <= 1e-4
         fa, fb = a.bonds()[1], b.bonds()[1]
         assert abs(int(fa.sum()) - int(fb.sum())) <= max(3, 0.01 * fb.sum())
 
