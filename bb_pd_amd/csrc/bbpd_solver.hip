@@ -64,6 +64,7 @@ constexpr int ST_ROW = 80;     // sites of a staged stencil row: 64 + 2 * ST_RMA
 // the rigid-body projection, so the ADR Rayleigh quotient leaves the node out)
 constexpr unsigned char CONTACT_BIT = 1u << 3;
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x8 __attribute__((ext_vector_type(8)));
 
@@ -173,6 +174,8 @@ struct Params {
     const int2 *srow;     // per stencil row (offsets sharing dy, dz): offset count, slot delta at dx = 0;
                           // ST_PAD entries past each region's last
     const uint4 *smask;   // per slot: bit o set when the node has the bond at offset o
+    const unsigned long long *smaskT; // per (slice, offset) in entry order: bit lane set when the slot
+                                      // (slice, lane) has the bond; ST_PAD entries past the last
     void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
     const float *tabf;    // mixed-form key tables (c * V, then parameter tables)
 };
@@ -528,6 +531,7 @@ struct StEnt {
     T h;      // stored history (sign = flag)
     T p0, p1; // streamed s0, sc
     StOff st; // per-offset data (the same for every lane)
+    unsigned long long mk; // the wavefront's lanes that have the bond
 };
 
 // one stencil row of neighbour records in flight: sites 0 .. 63 (a) and 64 .. 63 + 2 rx (b)
@@ -542,8 +546,10 @@ using sconst_ptr = const __attribute__((address_space(4))) T *;
 template <typename T>
 __device__ __forceinline__ T sload(const T *p, int i)
 {
-    static_assert(sizeof(T) == 4 || sizeof(T) == 16 || sizeof(T) == 32, "32-, 128- or 256-bit elements");
-    if constexpr (sizeof(T) == 32)
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16 || sizeof(T) == 32, "32- to 256-bit elements");
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, ((sconst_ptr<u32x2>)p)[i]);
+    else if constexpr (sizeof(T) == 32)
         return __builtin_bit_cast(T, ((sconst_ptr<u32x8>)p)[i]);
     else if constexpr (sizeof(T) == 16)
         return __builtin_bit_cast(T, ((sconst_ptr<u32x4>)p)[i]);
@@ -556,7 +562,6 @@ __device__ __forceinline__ T sload(const T *p, int i)
 // fixed costs no vector instruction per access; a read at a per-lane offset past num_records
 // returns zeros without touching memory.
 using brsrc = __amdgpu_buffer_rsrc_t;
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int CP_NT = 2; // cache policy: non-temporal
 
 __device__ __forceinline__ brsrc make_rsrc(const void *base, uint32_t bytes)
@@ -673,7 +678,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const int krow = gme * kg * KSCALE;      // the node's row of the key table (index or bytes)
     // X == h * k exactly (checked at set-up)
     const double Xn0 = P.hx * (double)kx, Xn1 = P.hy * (double)ky, Xn2 = P.hz * (double)kz;
-    const uint4 msk = P.smask[n];
     // this wavefront's slice of the entry arrays: entry (o, lane) at byte (o * SLICE + lane) * TW
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / SLICE));
     const long long eslice = RG.e0 + ((long long)((gb * BLOCK - base) / SLICE) + wv) * noff * SLICE;
@@ -716,6 +720,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     };
     double f0 = 0, f1 = 0, f2 = 0;
     const StOff *sst = P.sst + RG.sst0;
+    const unsigned long long *mks = P.smaskT + eslice / SLICE;
     // offset o's loads; o runs to noff: the entry arrays end at least two slices' worth past the
     // last entry and the per-offset table ST_PAD records past the last offset
     auto load = [&](StEnt<T> &d, int o) {
@@ -724,6 +729,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         if (PV == PV_GENERAL && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lb, so);
         if (PV == PV_GENERAL && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lb, so);
         d.st = sload(sst, o);
+        d.mk = sload(mks, o);
     };
     // One entry.  The FP64 form adds its force to the row sum here; the mixed form adds it to the
     // FP32 sum `a` of a pair of entries (offsets o, o + 1), which the loop adds to the FP64 row sum.
@@ -846,24 +852,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         };
         StEnt<T> A, B;
         load(A, 0);
-        for (int c = 0; c < noff; c += 32) {
-            // bond-mask bits of offsets c .. c + 31, shifted as o advances
-            uint32_t mw = c == 0 ? msk.x : (c == 32 ? msk.y : (c == 64 ? msk.z : msk.w));
-            const int oe = min(c + 32, noff);
-            for (int o = c; o < oe; o += 2) {
-                float3 a = make_float3(0.0f, 0.0f, 0.0f);
-                load(B, o + 1);
-                next_row(o);
-                compute(A, o, (mw & 1u) != 0u, a);
-                load(A, o + 2);
-                next_row(o + 1);
-                compute(B, o + 1, (mw & 2u) != 0u, a);
-                if constexpr (MIXED) {
-                    f0 += (double)a.x;
-                    f1 += (double)a.y;
-                    f2 += (double)a.z;
-                }
-                mw >>= 2;
+        // the lane masks are wavefront-uniform: a lane's bit is tested by running the entry under
+        // the mask as the execution mask
+        for (int o = 0; o < noff; o += 2) {
+            float3 a = make_float3(0.0f, 0.0f, 0.0f);
+            load(B, o + 1);
+            next_row(o);
+            compute(A, o, __builtin_amdgcn_inverse_ballot_w64(A.mk), a);
+            load(A, o + 2);
+            next_row(o + 1);
+            compute(B, o + 1, __builtin_amdgcn_inverse_ballot_w64(B.mk), a);
+            if constexpr (MIXED) {
+                f0 += (double)a.x;
+                f1 += (double)a.y;
+                f2 += (double)a.z;
             }
         }
     }
@@ -1670,6 +1672,19 @@ struct StencilPlan {
     int s0mode = PM_SCALAR, scmode = PM_SCALAR;
 };
 
+// Lane masks of the stencil layout: per slice of 64 slots and per offset, bit lane set when the slot
+// (slice, lane) has the bond at that offset; from the per-slot masks `m` of `nslots` slots.
+std::vector<unsigned long long> lane_masks(const std::vector<uint4> &m, long long nslots, int noff)
+{
+    std::vector<unsigned long long> r((size_t)(nslots / SLICE) * noff, 0ull);
+    for (long long q = 0; q < nslots; q++) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(&m[q]);
+        for (int o = 0; o < noff; o++)
+            if ((w[o >> 5] >> (o & 31)) & 1u) r[(size_t)(q / SLICE) * noff + o] |= 1ull << (q % SLICE);
+    }
+    return r;
+}
+
 // Per-offset records of the stencil layout: xi = h * d rounded to FP32 per component (an offset and
 // its negation give exactly negated vectors), |xi| from those, the volume correction factor of
 // |h * d| (calculatevolumecorrectionfactors.m:20-32), and d; then ST_PAD copies of the last.
@@ -2130,6 +2145,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     // stencil layout
     void *hsd = nullptr;
     uint4 *maskd = nullptr;
+    unsigned long long *maskTd = nullptr;
     int4 *stoffd = nullptr;
     int2 *srowd = nullptr;
     StOff *sstd = nullptr;
@@ -2137,6 +2153,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     if (sp.ok) {
         if ((rc = dalloc((char **)&hsd, (size_t)(nE + 2 * SLICE) * (mixed ? 4 : 8), s->owned))) return fail(rc);
         if ((rc = dalloc(&maskd, nSlots, s->owned))) return fail(rc);
+        if ((rc = dalloc(&maskTd, (size_t)s->nSlices * sp.noff + ST_PAD, s->owned))) return fail(rc);
         if ((rc = dalloc(&stoffd, sp.noff, s->owned))) return fail(rc);
         if ((rc = dalloc(&srowd, sp.rows.size(), s->owned))) return fail(rc);
         if ((rc = dalloc(&sstd, sp.noff + ST_PAD, s->owned))) return fail(rc);
@@ -2218,6 +2235,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
             setbit(s->slot_of[bj[k]], sp.oj[k]);
         }
         BBPD_HIP(hipMemcpy(maskd, hm.data(), sizeof(uint4) * nSlots, hipMemcpyHostToDevice));
+        const std::vector<unsigned long long> hmt = lane_masks(hm, nSlots, sp.noff);
+        BBPD_HIP(hipMemcpy(maskTd, hmt.data(), sizeof(unsigned long long) * hmt.size(), hipMemcpyHostToDevice));
         if (mixed) {
             std::vector<float> h0(nE, 0.0f);
             for (int k = 0; k < nB; k++)
@@ -2310,6 +2329,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.srow = srowd;
     P.sst = sstd;
     P.smask = maskd;
+    P.smaskT = maskTd;
     P.hs = hsd;
     P.tabf = tabfd;
     // key tables in LDS; with s0 and sc both tabled the stencil kernel keeps (c, law threshold) per key
@@ -3169,6 +3189,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     NodeRec *posd[2];
     unsigned char *consd;
     uint4 *maskd;
+    unsigned long long *maskTd;
     void *hsd, *s0d = nullptr, *scd = nullptr;
     int4 *stoffd;
     int2 *srowd;
@@ -3186,6 +3207,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if (sh->integrator == BBPD_INTEGRATOR_ADR && (rc = dalloc(&massd, nS, s->owned))) return fail(rc);
     if ((rc = dalloc(&consd, nS, s->owned))) return fail(rc);
     if ((rc = dalloc(&maskd, nS, s->owned))) return fail(rc);
+    if ((rc = dalloc(&maskTd, nS / SLICE * ST_MAX + ST_PAD, s->owned))) return fail(rc);
     if ((rc = dalloc((char **)&hsd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if (sh->s0 && (rc = dalloc((char **)&s0d, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if (sh->sc && (rc = dalloc((char **)&scd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
@@ -3281,6 +3303,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     P.srow = srowd;
     P.sst = sstd;
     P.smask = maskd;
+    P.smaskT = maskTd;
     P.hs = hsd;
     s->device_bytes = nS * (2 * sizeof(NodeRec) + 8 * (3 * (mixed ? 3 : 4) + 2 + (Fpd ? 3 : 0)) + 1 + 16) +
                       (size_t)nE * ew * (1 + (s0d ? 1 : 0) + (scd ? 1 : 0));
@@ -3512,6 +3535,11 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     if (P.mass) BBPD_HIP(hipMemcpy((double *)P.mass + base, hm.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy((unsigned char *)P.cons + base, hc.data(), nSl, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy((uint4 *)P.smask + base, hmask.data(), sizeof(uint4) * nSl, hipMemcpyHostToDevice));
+    {
+        const std::vector<unsigned long long> hmt = lane_masks(hmask, nSl, noff);
+        BBPD_HIP(hipMemcpy((unsigned long long *)P.smaskT + RG.e0 / SLICE, hmt.data(),
+                           sizeof(unsigned long long) * hmt.size(), hipMemcpyHostToDevice));
+    }
     auto put_entries = [&](void *dst, const std::vector<double> &v) -> int {
         if (mixed) {
             std::vector<float> f(v.size());
