@@ -988,21 +988,49 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
         const double disp = schedule_disp(RG, it);
         const double cx = B.cx, czm = B.cz + (B.moves ? disp : 0.0), R = B.R;
         const int nf = B.nFam;
-        for (int base = 0; base < nf; base += blockDim.x) {
-            const int t = base + threadIdx.x;
-            bool hit = false;
-            double d_z = 0;
-            if (t < nf) {
-                const int i = B.fam[t];
-                NodeRec pr = pos[i];
-                double pv[3] = {pr.x, pr.y, pr.z};
-                double xc[3];
+        double part = 0; // body 0: this thread's share of the static reaction, members in order
+        // the members in groups of SE_CH chunks of blockDim.x: every load of a group is issued
+        // before any of its members is projected, so a family costs about one memory latency per
+        // group rather than one chain of dependent loads per chunk
+        constexpr int SE_CH = 4;
+        for (int g0 = 0; g0 < nf; g0 += SE_CH * (int)blockDim.x) {
+            int idx[SE_CH];
+            NodeRec pr[SE_CH];
+            double x0[SE_CH], x2[SE_CH], fz[SE_CH];
+            bool hit[SE_CH];
+            double d_z[SE_CH];
 #pragma unroll
-                for (int d = 0; d < 3; d++) xc[d] = MIXED ? XX[d][i] + pv[d] : pv[d];
+            for (int k = 0; k < SE_CH; k++) {
+                const int t = g0 + k * (int)blockDim.x + (int)threadIdx.x;
+                idx[k] = t < nf ? B.fam[t] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < SE_CH; k++)
+                if (idx[k] >= 0) {
+                    pr[k] = pos[idx[k]];
+                    if (bi == 0) fz[k] = P.F2[idx[k]];
+                    if (MIXED) {
+                        x0[k] = XX[0][idx[k]];
+                        x2[k] = XX[2][idx[k]];
+                    }
+                }
+#pragma unroll
+            for (int k = 0; k < SE_CH; k++) {
+                hit[k] = false;
+                d_z[k] = 0;
+                if (idx[k] < 0) continue;
+                const int i = idx[k];
+                // calculatereactionforceFast.m:3-5 over body 0's family: F_z * cellVolume
+                if (bi == 0) part = part + fz[k] * P.V;
+                double pv[3] = {pr[k].x, pr[k].y, pr[k].z};
+                double xc[3];
+                xc[0] = MIXED ? x0[k] + pv[0] : pv[0];
+                xc[2] = MIXED ? x2[k] + pv[2] : pv[2];
                 const double ddx = xc[0] - cx, ddz = xc[2] - czm;
                 const double dist = sqrt((ddx * ddx) + (ddz * ddz));
                 if (dist < R) {
-                    hit = true;
+                    hit[k] = true;
+                    xc[1] = MIXED ? XX[1][i] + pv[1] : pv[1];
                     xc[0] = cx + (ddx / dist) * R;
                     xc[2] = czm + (ddz / dist) * R;
 #pragma unroll
@@ -1018,7 +1046,7 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
                             pv[d] = xc[d];
                         }
                         vv[d][i] = nv;
-                        if (d == 2) d_z = -1 * P.rho[i] * (nv - vo) / P.DT * P.V;
+                        if (d == 2) d_z[k] = -1 * P.rho[i] * (nv - vo) / P.DT * P.V;
                     }
                     pos[i].x = pv[0];
                     pos[i].y = pv[1];
@@ -1027,26 +1055,24 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
             }
             // compact the members in contact, in family order, then sum them in that order
             // (the order of MATLAB's sum over dpenfz; members not in contact add nothing)
-            const unsigned long long mask = __ballot(hit);
-            if (lane == 0) wofs[wv + 1] = __popcll(mask);
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                wofs[0] = 0;
-                for (int q = 1; q <= nw; q++) wofs[q] += wofs[q - 1];
+            for (int k = 0; k < SE_CH && g0 + k * (int)blockDim.x < nf; k++) {
+                const unsigned long long mask = __ballot(hit[k]);
+                if (lane == 0) wofs[wv + 1] = __popcll(mask);
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    wofs[0] = 0;
+                    for (int q = 1; q <= nw; q++) wofs[q] += wofs[q - 1];
+                }
+                __syncthreads();
+                if (hit[k]) vals[wofs[wv] + __popcll(mask & ((1ull << lane) - 1))] = d_z[k];
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    for (int q = 0; q < wofs[nw]; q++) total = total + vals[q];
+                __syncthreads();
             }
-            __syncthreads();
-            if (hit) vals[wofs[wv] + __popcll(mask & ((1ull << lane) - 1))] = d_z;
-            __syncthreads();
-            if (threadIdx.x == 0)
-                for (int q = 0; q < wofs[nw]; q++) total = total + vals[q];
-            __syncthreads();
         }
-        if (bi == 0) {
-            // calculatereactionforceFast.m:3-5 over body 0's family: F_z * cellVolume (fixed tree)
-            double part = 0;
-            for (int t = threadIdx.x; t < nf; t += blockDim.x) part = part + P.F2[B.fam[t]] * P.V;
-            rs = bbpd::block_sum_fixed<1024>(part, vals);
-        }
+        // the static reaction: the threads' shares summed in a fixed tree
+        if (bi == 0) rs = bbpd::block_sum_fixed<1024>(part, vals);
     }
     if (threadIdx.x == 0) {
         if (bi >= 0) RS.reaction[bi] = total;
