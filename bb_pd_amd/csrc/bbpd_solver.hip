@@ -183,6 +183,19 @@ struct Params {
 // ------------------------------------------------------------------------------------------------
 // Softening laws, per entry.  Each mirrors its reference routine; `w` carries the flags.
 // ------------------------------------------------------------------------------------------------
+// a / b: the IEEE quotient in FP64; in FP32 (the mixed form) the product with the hardware
+// reciprocal (within 1 ulp of the FP32 reciprocal), which the mixed form's law takes
+template <typename T>
+__device__ __forceinline__ T ldiv(T a, T b)
+{
+    if constexpr (std::is_same<T, float>::value)
+        return a * __builtin_amdgcn_rcpf(b);
+    else
+        return a / b;
+}
+
+// The laws.  A bond whose history is already 1 (broken) keeps it: every candidate is clamped to 1
+// or does not exceed it, so the bilinear and exponential candidates are only computed below 1.
 template <int LAW, typename T>
 __device__ inline void apply_law(T s, T s0, T sc, T s1, T kk, T alpha, T denom, T sc_steel, T ff, T &H, uint32_t &w)
 {
@@ -197,8 +210,8 @@ __device__ inline void apply_law(T s, T s0, T sc, T s1, T kk, T alpha, T denom, 
         // calculatebondsofteningfactor.m:42-48.  With the flag clear the candidate is +-0 or NaN and
         // never raises H (kept in [0,1]), so the arithmetic only runs for flagged bonds.
         if (type == 0 && s > s0) w |= FLAG_BIT;
-        if (w & FLAG_BIT) {
-            T cand = ((s - s0) / s) * (sc / (sc - s0));
+        if ((w & FLAG_BIT) && H < T(1)) {
+            T cand = ldiv(s - s0, s) * ldiv(sc, sc - s0);
             if (cand == cand && cand > H) H = cand;
             if (H > T(1)) H = T(1);
         }
@@ -225,9 +238,11 @@ __device__ inline void apply_law(T s, T s0, T sc, T s1, T kk, T alpha, T denom, 
             T bsf = T(0);
             if (s0 < s && s <= sc) {
                 w |= FLAG_BIT;
-                T numerator = T(1) - exp(-kk * (s - s0) / (sc - s0));
-                T residual = alpha * (T(1) - (s - s0) / (sc - s0));
-                bsf = T(1) - (s0 / s) * ((T(1) - (numerator / denom)) + residual) / (T(1) + alpha);
+                if (H < T(1)) {
+                    T numerator = T(1) - exp(ldiv(-kk * (s - s0), sc - s0));
+                    T residual = alpha * (T(1) - ldiv(s - s0, sc - s0));
+                    bsf = T(1) - ldiv(ldiv(s0, s) * ((T(1) - ldiv(numerator, denom)) + residual), T(1) + alpha);
+                }
             } else if (s > sc) {
                 bsf = T(1);
             }
