@@ -15,7 +15,9 @@ Reported (one JSON line on stdout):
   roofline         dominant kernel (bond kernel, mixed form): the byte model of the hot-path survey,
                    E*(4 + w_state + w_kappa + n_par*w) + N*180 with E = 2*nBonds entries, divided by
                    the kernel's average launch time from HIP events on the solver's stream, against
-                   8 TB/s; `traffic` is the kernel's measured HBM bytes per launch (rocprofv3
+                   8 TB/s (a fraction above 1 means the stencil layout moves fewer bytes than that
+                   model); `frac_layout_bytes` is the same with the stencil layout's own bytes per
+                   launch; `traffic` is the kernel's measured HBM bytes per launch (rocprofv3
                    FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for a device
                    copy of known size)
   cpu_baseline     the oracle's OpenMP restatement of the reference CPU passes timed on the host on
@@ -520,6 +522,9 @@ def main():
         "precision": prec,
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["frac"], "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_fetch": traffic["fetch_bytes"] if traffic else None,
+                     "traffic_write": traffic["write_bytes"] if traffic else None,
+                     "frac_layout_bytes": head["bytes_layout"] / (head["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "kernel": f"bond kernel, FP32-mixed, {head['layout']} layout", "kernel_ms": head["kernel_ms"],
                      "autotune": {"variant": head["autotune"][0], "ms": head["autotune"][1]} if head["autotune"] else None,
                      "bytes_model": head["bytes_model"], "bytes_layout": head["bytes_layout"]},

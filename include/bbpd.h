@@ -250,8 +250,8 @@ int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel
                      int64_t *n_padded_entries);
 /* the bond-table layout chosen at set-up: "stencil" (full lattice box, rows in lattice-offset
  * order: neighbours implicit, per entry only the history) or "records (<why not stencil>)".  The
- * string lives as long as the solver.  Default: stencil where it applies for the FP32-mixed form,
- * records for plain FP64; BBPD_LAYOUT=stencil or BBPD_LAYOUT=records in the environment chooses. */
+ * string lives as long as the solver.  Default: stencil where it applies, for both forms;
+ * BBPD_LAYOUT=records in the environment asks for the record layout. */
 const char *bbpd_solver_layout(bbpd_solver *s);
 /* Autotuner: times each compiled launch variant of the stencil bond kernel over `reps` launches on
  * the current state and keeps the fastest for the solver or batch.  Variants: 0 = compiler's
