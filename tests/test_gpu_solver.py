@@ -334,6 +334,29 @@ def test_stencil_layout_equals_records(gpu, law):
             assert oa["n_softening"] > 0
 
 
+@pytest.mark.parametrize("horizon_factor", [4.6, 5.9])
+def test_stencil_layout_wide_rows(gpu, horizon_factor):
+    """A one-node-thick plate with a horizon of 4.6 or 5.9 spacings: bond offsets reach 4 or 5
+    spacings along x, so each staged stencil row carries 8 or 10 halo sites beyond its 64 (the
+    usual horizon, pi * DX, reaches 3).  The stencil layout still equals the record layout bit for
+    bit in FP64."""
+    case = beam.build_beam("t", 30, 1, 12, 5e-3, law=_lib.LAW_BILINEAR, s0=1e-4, sc=2e-3,
+                           horizon_factor=horizon_factor, integrator=_lib.EULER_CROMER, schedule=0,
+                           nTimeSteps=150, appliedDisplacement=-1.5e-4)
+    print(f"case: plate, horizon {horizon_factor} dx, {case.nBonds} bonds", flush=True)
+    a = _solver_with_layout(case, _lib.FP64, "stencil")
+    b = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b:
+        assert a.layout() == "stencil", a.layout()
+        a.step(150)
+        b.step(150)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+        assert np.abs(a.nodal()[0]).max() > 0
+
+
 def test_stencil_layout_restore_and_adr(gpu):
     """A checkpoint of the stencil layout restores into the record layout (the history's sign-bit
     flag round-trips) and the two continue bit for bit under Euler-Cromer.  Under ADR the damping
