@@ -649,7 +649,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
 {
     using T = typename std::conditional<MIXED, float, double>::type;
     constexpr uint32_t TW = sizeof(T);
-    __shared__ double sh[BLOCK];
     // per wavefront, a row buffer: x, y, z and the key word (low word) of ST_ROW sites.  One buffer
     // suffices: a wavefront's LDS accesses complete in program order, so staging the next row
     // (after the last read of this one) cannot overtake a read.
@@ -884,7 +883,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             }
         }
     }
-    finish_node<MIXED>(P, n, gb, real, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, sh);
+    // the block reduction of finish_node (ADR) reuses the row buffers once every wavefront is past
+    // its rows: no separate scratch, so less LDS per block and more blocks per CU
+    if (P.adr) __syncthreads();
+    finish_node<MIXED>(P, n, gb, real, cur, f0, f1, f2, qn0, qn1, qn2, Xn0, Xn1, Xn2, &s_row[0][0]);
 }
 
 // ADR: damping coefficient of each active region from its blocks' partials
