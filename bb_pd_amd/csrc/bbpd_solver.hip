@@ -194,6 +194,17 @@ __device__ __forceinline__ T ldiv(T a, T b)
         return a / b;
 }
 
+// e^x: in FP64 the library exponential; in FP32 (the mixed form) 2^(x log2 e) by the hardware
+// exponential.  For the law's arguments (x in [-k, 0]) its absolute error stays at the FP32 level.
+template <typename T>
+__device__ __forceinline__ T lexp(T x)
+{
+    if constexpr (std::is_same<T, float>::value)
+        return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    else
+        return exp(x);
+}
+
 // The laws.  A bond whose history is already 1 (broken) keeps it: every candidate is clamped to 1
 // or does not exceed it, so the bilinear and exponential candidates are only computed below 1.
 template <int LAW, typename T>
@@ -239,7 +250,7 @@ __device__ inline void apply_law(T s, T s0, T sc, T s1, T kk, T alpha, T denom, 
             if (s0 < s && s <= sc) {
                 w |= FLAG_BIT;
                 if (H < T(1)) {
-                    T numerator = T(1) - exp(ldiv(-kk * (s - s0), sc - s0));
+                    T numerator = T(1) - lexp(ldiv(-kk * (s - s0), sc - s0));
                     T residual = alpha * (T(1) - ldiv(s - s0, sc - s0));
                     bsf = T(1) - ldiv(ldiv(s0, s) * ((T(1) - ldiv(numerator, denom)) + residual), T(1) + alpha);
                 }
@@ -934,17 +945,23 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
     double *FP[3] = {P.Fp0, P.Fp1, P.Fp2};
     const double *XX[3] = {P.X0, P.X1, P.X2};
     double *uu[3] = {P.u0, P.u1, P.u2};
-    const NodeRec pc = P.pos[cur][n];
-    const double pcv[3] = {pc.x, pc.y, pc.z};
-    double out[3];
+    const NodeRec *pc = P.pos[cur] + n; // the position only (24 of the record's 32 bytes)
+    const double pcv[3] = {pc->x, pc->y, pc->z};
+    // every load before any store (the arrays are distinct, which the compiler cannot assume)
+    double f[3], vo[3], uold[3], xr[3], out[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double f = FF[d][n];
+        f[d] = FF[d][n];
+        vo[d] = vv[d][n];
+        uold[d] = MIXED ? pcv[d] : uu[d][n];
+        xr[d] = MIXED ? 0.0 : XX[d][n];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
         double nv;
-        if (it == 1) nv = 0.5 * DT * minv * f;
-        else nv = (((2 - cn * DT) * vv[d][n]) + (2 * DT * minv * f)) / (2 + cn * DT);
-        const double uold = MIXED ? pcv[d] : uu[d][n];
-        double nu = uold + (DT * nv);
+        if (it == 1) nv = 0.5 * DT * minv * f[d];
+        else nv = (((2 - cn * DT) * vo[d]) + (2 * DT * minv * f[d])) / (2 + cn * DT);
+        double nu = uold[d] + (DT * nv);
         if (cf & (1u << d)) {
             nu = 0;
             nv = 0;
@@ -954,9 +971,9 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
             out[d] = nu;
         } else {
             uu[d][n] = nu;
-            out[d] = XX[d][n] + nu;
+            out[d] = xr[d] + nu;
         }
-        FP[d][n] = f;
+        FP[d][n] = f[d];
     }
     NodeRec *po = P.pos[nxt] + n;
     po->x = out[0];
