@@ -1505,7 +1505,7 @@ KernelFn pick_occ(int v)
     switch (v) {
     case 1: return k_bond_st<MIXED, LAW, 1, false, PV>;
     case 2: return k_bond_st<MIXED, LAW, 5, true, PV>;
-    case 3: return k_bond_st<MIXED, LAW, 8, true, PV>;
+    case 3: return k_bond_st<MIXED, LAW, 5, false, PV>;
     default: return k_bond_st<MIXED, LAW, 1, true, PV>;
     }
 }

@@ -57,10 +57,13 @@ def survey_model_bytes(case, mixed: bool) -> int:
     return E * (4 + w + w + n_par * w) + case.nNodes * 180
 
 
-def run_form(case, precision, steps, warmup, chunk=None, keep=()):
+def run_form(case, precision, steps, warmup, chunk=None, keep=(), retune=False):
     """Build the solver, run `warmup` untimed steps, then exactly `steps` timed steps in chunks of
     `chunk` (each bracketed by synchronisations; the times are summed).  Between chunks, untimed,
-    the state at the step counts in `keep` is checkpointed to host memory.  Returns the solver (open)
+    the state at the step counts in `keep` is checkpointed to host memory.  With `retune` (ADR on the
+    stencil layout) the autotuner reruns at the start of every chunk, inside the timed window, so
+    the launch variant follows the load history (XCD-contiguous block order keeps neighbour rows in
+    one L2; launch order spreads a localised process zone over the XCDs).  Returns the solver (open)
     with the timing, the whole per-step history from step 1 and the checkpoints."""
     import torch
     from bb_pd_amd import _lib
@@ -77,9 +80,11 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=()):
     if warmup in keep:
         ckpts[warmup] = sol.checkpoint()
     tune = None
-    if case.integrator == _lib.ADR and sol.layout() == "stencil":
+    tunable = case.integrator == _lib.ADR and sol.layout() == "stencil"
+    if tunable:
         tune = sol.autotune(5)
         log(f"  autotune: variant {tune[0]} of {['%.3f' % x for x in tune[1]]} ms")
+    picks = []
     sol.set_timing(True)
     dt, done = 0.0, 0
     while done < steps:
@@ -88,6 +93,8 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=()):
         n = min((at // chunk + 1) * chunk - at, steps - done)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        if retune and tunable and done > 0:
+            picks.append(sol.autotune(3)[0])
         sol.step(n)
         sol.sync()
         torch.cuda.synchronize()
@@ -108,7 +115,7 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=()):
     return dict(solver=sol, seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=case.nBonds * steps / dt,
                 kernel_ms=avg_ms, launches=launches, bytes_model=model, bytes_layout=info["bond_kernel_bytes"],
                 achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, info=info, hist=hist, ckpts=ckpts,
-                layout=sol.layout(), autotune=tune)
+                layout=sol.layout(), autotune=tune, retune_picks=picks)
 
 
 def increment_curve(R: np.ndarray, ipi: int) -> np.ndarray:
@@ -457,14 +464,14 @@ def main():
     study = args.form == "both" and not args.no_precision and total >= 2 * ipi
     # FP64 states kept for the precision study: after increment 1 and every fifth increment
     keep = sorted({ipi * j for j in range(1, total // ipi + 1) if j == 1 or j % 5 == 0}) if study else ()
-    chunk = ipi if study else None
+    chunk = ipi  # load increments: checkpoints for the precision study, and autotuner reruns
     fp64 = mixed = None
     if args.form in ("both", "fp64"):
-        fp64 = run_form(case, _lib.FP64, args.steps, args.warmup, chunk, keep)
+        fp64 = run_form(case, _lib.FP64, args.steps, args.warmup, chunk, keep, retune=True)
         log(f"fp64: {fp64['ms_per_step']:.3f} ms/step, kernel {fp64['kernel_ms']:.3f} ms, "
             f"{fp64['achieved_gbs']:.0f} GB/s (model bytes)")
     if args.form in ("both", "mixed"):
-        mixed = run_form(case, _lib.MIXED, args.steps, args.warmup, chunk)
+        mixed = run_form(case, _lib.MIXED, args.steps, args.warmup, chunk, retune=True)
         log(f"mixed: {mixed['ms_per_step']:.3f} ms/step, kernel {mixed['kernel_ms']:.3f} ms, "
             f"{mixed['achieved_gbs']:.0f} GB/s (model bytes)")
     if args.form != "both":
@@ -516,7 +523,8 @@ def main():
         },
         "fp64": {"value": fp64["bond_updates_per_s"], "ms_per_step": fp64["ms_per_step"],
                  "kernel_ms": fp64["kernel_ms"], "bytes_model": fp64["bytes_model"],
-                 "bytes_layout": fp64["bytes_layout"], "achieved_gbs": fp64["achieved_gbs"], "frac": fp64["frac"]},
+                 "bytes_layout": fp64["bytes_layout"], "achieved_gbs": fp64["achieved_gbs"], "frac": fp64["frac"],
+                 "retune_picks": {str(v): fp64["retune_picks"].count(v) for v in sorted(set(fp64["retune_picks"]))}},
         "mixed_over_fp64": head["bond_updates_per_s"] / fp64["bond_updates_per_s"],
         "mixed_vs_fp64_load_err": prec["same_state_peak_load_err"] if prec else None,
         "precision": prec,
@@ -526,7 +534,10 @@ def main():
                      "traffic_write": traffic["write_bytes"] if traffic else None,
                      "frac_layout_bytes": head["bytes_layout"] / (head["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "kernel": f"bond kernel, FP32-mixed, {head['layout']} layout", "kernel_ms": head["kernel_ms"],
-                     "autotune": {"variant": head["autotune"][0], "ms": head["autotune"][1]} if head["autotune"] else None,
+                     "autotune": {"variant": head["autotune"][0], "ms": head["autotune"][1],
+                                  "retune_picks": {str(v): head["retune_picks"].count(v)
+                                                   for v in sorted(set(head["retune_picks"]))}}
+                                 if head["autotune"] else None,
                      "bytes_model": head["bytes_model"], "bytes_layout": head["bytes_layout"]},
         "cpu_baseline": cpu,
     }
