@@ -136,7 +136,7 @@ struct Params {
     double *u0, *u1, *u2; // displacement (FP64 form only)
     double *v0, *v1, *v2;
     double *F0, *F1, *F2;
-    double *Fp0, *Fp1, *Fp2; // ADR previous force
+    double *Fp0, *Fp1, *Fp2; // ADR: the second force set (see adr_force)
     const double *rho, *mass;
     const unsigned char *cons; // bit d set = DOF d constrained; CONTACT_BIT = contact-family member
     // entries
@@ -276,6 +276,16 @@ __device__ inline uint32_t *rec_word(uint4 *rec, long long e, int word)
     return reinterpret_cast<uint32_t *>(rec + e) + word;
 }
 
+// ADR keeps the forces of the last two steps in the two force sets, by launch parity: the launch with
+// parity `cur` writes its F to set `cur` and reads the previous step's from set `cur ^ 1` (so nothing
+// copies F to F_prev).  Euler-Cromer uses set 0 only.
+__device__ __forceinline__ double *adr_force(const Params &P, int set, int d)
+{
+    double *const a[3] = {P.F0, P.F1, P.F2};
+    double *const b[3] = {P.Fp0, P.Fp1, P.Fp2};
+    return set ? b[d] : a[d];
+}
+
 // After a node's row: the Euler-Cromer update of the node (timeintegrationeulercromerMex.c:38-46),
 // writing its next position record, or, for ADR, F and the block's Rayleigh-quotient partial sums
 // (calculatedampingcoefficient.m:61-85).  Every thread of the block calls it (block reduction).
@@ -319,9 +329,9 @@ __device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool
     } else {
         double c1 = 0, c2 = 0;
         if (real) {
-            P.F0[n] = f0;
-            P.F1[n] = f1;
-            P.F2[n] = f2;
+            adr_force(P, cur, 0)[n] = f0;
+            adr_force(P, cur, 1)[n] = f1;
+            adr_force(P, cur, 2)[n] = f2;
         }
         // the quotient runs over the nodes no contact body drives: a family member's velocity
         // after a projection is the projection's correction, and its term would outweigh the
@@ -329,7 +339,8 @@ __device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool
         if (real && !(P.cons[n] & CONTACT_BIT)) {
             const double mi = P.mass[n];
             const double ff[3] = {f0, f1, f2};
-            const double fp[3] = {P.Fp0[n], P.Fp1[n], P.Fp2[n]};
+            const double fp[3] = {adr_force(P, cur ^ 1, 0)[n], adr_force(P, cur ^ 1, 1)[n],
+                                  adr_force(P, cur ^ 1, 2)[n]};
             const double vp[3] = {P.v0[n], P.v1[n], P.v2[n]};
             double uq[3];
             if constexpr (MIXED) {
@@ -925,8 +936,8 @@ __global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P)
     }
 }
 
-// ADR: leapfrog update (timeintegrationADR.m:53-80), then x = X + u and F_prev = F
-// (the hand-over of dynamicsolverdisplacementcontrolledADR.m:153-155)
+// ADR: leapfrog update (timeintegrationADR.m:53-80), then x = X + u.  The hand-over F_prev = F
+// (dynamicsolverdisplacementcontrolledADR.m:153-155) is the parity of the force sets (adr_force).
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
 {
@@ -941,8 +952,7 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
     const double minv = 1 / P.mass[n];
     const unsigned char cf = P.cons[n];
     double *vv[3] = {P.v0, P.v1, P.v2};
-    double *FF[3] = {P.F0, P.F1, P.F2};
-    double *FP[3] = {P.Fp0, P.Fp1, P.Fp2};
+    const double *FF[3] = {adr_force(P, cur, 0), adr_force(P, cur, 1), adr_force(P, cur, 2)};
     const double *XX[3] = {P.X0, P.X1, P.X2};
     double *uu[3] = {P.u0, P.u1, P.u2};
     const NodeRec *pc = P.pos[cur] + n; // the position only (24 of the record's 32 bytes)
@@ -973,7 +983,6 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
             uu[d][n] = nu;
             out[d] = xr[d] + nu;
         }
-        FP[d][n] = f[d];
     }
     NodeRec *po = P.pos[nxt] + n;
     po->x = out[0];
@@ -1042,7 +1051,7 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
             for (int k = 0; k < SE_CH; k++)
                 if (idx[k] >= 0) {
                     pr[k] = pos[idx[k]];
-                    if (bi == 0) fz[k] = P.F2[idx[k]];
+                    if (bi == 0) fz[k] = P.adr ? adr_force(P, nxt ^ 1, 2)[idx[k]] : P.F2[idx[k]];
                     if (MIXED) {
                         x0[k] = XX[0][idx[k]];
                         x2[k] = XX[2][idx[k]];
@@ -2818,6 +2827,19 @@ int current_buffer(bbpd_solver *s, int *rc)
     return (int)(t & 1);
 }
 
+// The force arrays as seen between launches, `cur` = current_buffer: F = the last step's force and
+// Fp = the one before.  Under ADR the two sets alternate by launch parity (the launch with parity c
+// writes set c), so F is set cur ^ 1 and Fp set cur; Euler-Cromer keeps F in set 0.
+void force_sets(const bbpd_solver *s, int cur, double **F, double **Fp)
+{
+    double *const set[2][3] = {{s->P.F0, s->P.F1, s->P.F2}, {s->P.Fp0, s->P.Fp1, s->P.Fp2}};
+    const int f = s->P.adr ? (cur ^ 1) : 0;
+    for (int d = 0; d < 3; d++) {
+        F[d] = set[f][d];
+        Fp[d] = set[f ^ 1][d];
+    }
+}
+
 } // namespace
 
 extern "C" int bbpd_solver_get_nodal(bbpd_solver *s, double *u, double *v, double *F, double *x)
@@ -2831,7 +2853,8 @@ extern "C" int bbpd_solver_get_nodal(bbpd_solver *s, double *u, double *v, doubl
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
     double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
-    double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
+    double *fc[3], *fpc[3];
+    force_sets(s, cur, fc, fpc);
     if (u && (rc = mixed ? fetch_pos(s, cur, u) : fetch_soa(s, uc, u))) return rc;
     if (v && (rc = fetch_soa(s, vc, v))) return rc;
     if (F && (rc = fetch_soa(s, fc, F))) return rc;
@@ -2913,7 +2936,10 @@ extern "C" int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, 
     if ((rc = bbpd_solver_get_nodal(s, u, v, F, x))) return rc;
     if (Fp) {
         if (s->P.Fp0) {
-            double *fp[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
+            const int cur = current_buffer(s, &rc);
+            if (rc) return rc;
+            double *fc[3], *fp[3];
+            force_sets(s, cur, fc, fp);
             if ((rc = fetch_soa(s, fp, Fp))) return rc;
         } else {
             std::memset(Fp, 0, sizeof(double) * 3 * (size_t)s->nN);
@@ -2935,8 +2961,8 @@ extern "C" int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u
     if (rc) return rc;
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2};
     double *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
-    double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
-    double *fpc[3] = {s->P.Fp0, s->P.Fp1, s->P.Fp2};
+    double *fc[3], *fpc[3];
+    force_sets(s, cur, fc, fpc);
     if (u && (rc = mixed ? put_pos(s, cur, u) : put_soa(s, uc, u))) return rc;
     if (!mixed && x && (rc = put_pos(s, cur, x))) return rc;
     if (v && (rc = put_soa(s, vc, v))) return rc;
@@ -3712,7 +3738,8 @@ extern "C" int bbpd_batch_get_nodal(bbpd_solver *s, int region, double *u, doubl
         return d == 0 ? q.x : (d == 1 ? q.y : q.z);
     };
     double *uc[3] = {s->P.u0, s->P.u1, s->P.u2}, *vc[3] = {s->P.v0, s->P.v1, s->P.v2};
-    double *fc[3] = {s->P.F0, s->P.F1, s->P.F2};
+    double *fc[3], *fpc[3];
+    force_sets(s, cur, fc, fpc);
     double *Xc[3] = {(double *)s->P.X0, (double *)s->P.X1, (double *)s->P.X2};
     if (u) {
         if (mixed) {
