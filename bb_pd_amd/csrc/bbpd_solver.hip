@@ -607,6 +607,15 @@ __device__ __forceinline__ brsrc make_rsrc(const void *base, uint32_t bytes)
 }
 
 template <typename T>
+__device__ __forceinline__ T bload(brsrc r, uint32_t voff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+
+template <typename T>
 __device__ __forceinline__ T bload_nt(brsrc r, uint32_t voff, uint32_t soff)
 {
     if constexpr (sizeof(T) == 4)
@@ -702,6 +711,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     }
     __syncthreads();
     const T *tab = tdst;
+    // the region's key tables in global memory (PV_TABLE reads s0 and sc there when a law runs)
+    const brsrc rtab = make_rsrc(tsrc + tab0, (uint32_t)ntab * TW);
     // a neighbour's staged key word: its family-size index, or (PV_SCALAR / PV_TABLE) that index's
     // byte offset in a row of the key table, so one add gives the entry's table address
     constexpr int KSCALE = PV == PV_GENERAL ? 1 : (PV == PV_TABLE ? 2 : 1) * (int)sizeof(T);
@@ -802,9 +813,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         auto law = [&](T s) {
             if (!law_active<LAW, T>(s, thr, flag)) return;
             if (PV == PV_TABLE) {
-                const int ki = key / KSCALE;
-                s0 = tsrc[tab0 + ts0 + ki];
-                sc = tsrc[tab0 + tsc + ki];
+                const uint32_t ki = (uint32_t)key / KSCALE;
+                s0 = bload<T>(rtab, (uint32_t)(ts0 + ki) * TW);
+                sc = bload<T>(rtab, (uint32_t)(tsc + ki) * TW);
             }
             uint32_t w = flag ? mark : 0u;
             apply_law<LAW, T>(s, s0, sc, rs1, lk, lalpha, ldenom, lsteel, lff, H, w);
