@@ -67,6 +67,8 @@ constexpr unsigned char CONTACT_BIT = 1u << 3;
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x8 __attribute__((ext_vector_type(8)));
+template <typename T>
+using T2 = T __attribute__((ext_vector_type(2)));
 
 // how a per-bond quantity reaches the kernel
 enum ParamMode { PM_SCALAR = 0, PM_TABLE = 1, PM_STREAM = 2, PM_INLINE = 3 };
@@ -122,7 +124,11 @@ struct __align__(32) NodeRec {
 struct __align__(32) StOff {
     float x, y, z, L0; // mixed form: xi = h * d rounded to FP32, and |xi|
     float vcf;         // mixed form: volume correction factor of |h * d|
-    int dx, dy, dz;    // the lattice offset d
+    float L0sq;        // mixed form: |xi|^2
+    int dx;            // the lattice offset d: dx, and dy | dz << 16 (16-bit fields)
+    int dyz;
+    __host__ __device__ int dy() const { return (int)(int16_t)(dyz & 0xFFFF); }
+    __host__ __device__ int dz() const { return dyz >> 16; }
 };
 constexpr int ST_PAD = 2; // per-offset entries past the last (copies of it): loads run two offsets ahead
 
@@ -633,6 +639,18 @@ __device__ __forceinline__ void bstore(T v, brsrc r, uint32_t voff, uint32_t sof
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
 
+// LDS byte address of a __shared__ object, and a pointer from such an address: an address that is
+// a whole sum needs no add in the instruction that reads through it
+__device__ __forceinline__ uint32_t lds_addr(const void *p)
+{
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+template <typename T>
+__device__ __forceinline__ const T *lds_ptr(uint32_t a)
+{
+    return (const T *)(const __attribute__((address_space(3))) T *)(uintptr_t)a;
+}
+
 struct RecWords {
     u32x4 a, b;
 };
@@ -714,15 +732,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     // the region's key tables in global memory (PV_TABLE reads s0 and sc there when a law runs)
     const brsrc rtab = make_rsrc(tsrc + tab0, (uint32_t)ntab * TW);
     // a neighbour's staged key word: its family-size index, or (PV_SCALAR / PV_TABLE) that index's
-    // byte offset in a row of the key table, so one add gives the entry's table address
+    // byte offset in a row of the key table; the node's row `krow` is then an LDS byte address, so one
+    // add gives the address of the entry's table record
     constexpr int KSCALE = PV == PV_GENERAL ? 1 : (PV == PV_TABLE ? 2 : 1) * (int)sizeof(T);
+    const int tabl = PV == PV_GENERAL ? 0 : (int)lds_addr(tabs);
 
     const NodeRec me = pos[n];
     const double qn0 = me.x, qn1 = me.y, qn2 = me.z;
     const int kx = (int)(me.q & LAT_MASK), ky = (int)((me.q >> LAT_BITS) & LAT_MASK),
               kz = (int)((me.q >> (2 * LAT_BITS)) & LAT_MASK);
     const int gme = (int)(me.q >> G_SHIFT); // index of the node's family size
-    const int krow = gme * kg * KSCALE;      // the node's row of the key table (index or bytes)
+    const int krow = gme * kg * KSCALE + tabl; // the node's row of the key table (index or LDS address)
     // X == h * k exactly (checked at set-up)
     const double Xn0 = P.hx * (double)kx, Xn1 = P.hy * (double)ky, Xn2 = P.hz * (double)kz;
     // this wavefront's slice of the entry arrays: entry (o, lane) at byte (o * SLICE + lane) * TW
@@ -779,7 +799,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         d.mk = sload(mks, o);
     };
     // One entry.  The FP64 form adds its force to the row sum here; the mixed form adds it to the
-    // FP32 sum `a` of a pair of entries (offsets o, o + 1), which the loop adds to the FP64 row sum.
+    // FP32 sum `a` of a group of up to four entries, which the loop adds to the FP64 row sum.
     auto compute = [&](const StEnt<T> &d, int o, bool bit, float3 &a) {
         if (!bit) return;
         const uint32_t so = (uint32_t)o * (SLICE * TW);
@@ -787,24 +807,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         const double nbx = nb[0], nby = nb[ST_ROW], nbz = nb[2 * ST_ROW];
         const int key = krow + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
         // the entry's key-table values: c (c * V in the mixed form), s0, sc
-        const T *tk = PV == PV_GENERAL ? tab + key : reinterpret_cast<const T *>(reinterpret_cast<const char *>(tab) + key);
+        const T *tk = PV == PV_GENERAL ? tab + key : lds_ptr<T>((uint32_t)key);
         constexpr uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
-        // the history is |h| (+0 for -0), the flag its sign bit
-        T H;
-        bool flag;
-        if constexpr (MIXED) {
-            const uint32_t hb = __float_as_uint(d.h);
-            H = __uint_as_float(hb & 0x7FFFFFFFu);
-            flag = (int)hb < 0;
-        } else {
-            const unsigned long long hb = (unsigned long long)__double_as_longlong(d.h);
-            H = __longlong_as_double((long long)(hb & 0x7FFFFFFFFFFFFFFFull));
-            flag = (long long)hb < 0;
-        }
-        T s0 = rs0, sc = rsc, thr = rthr;
+        // the stored value: the history is its magnitude (+0 for -0), the flag its sign bit; the law
+        // replaces it, and the force reads the magnitude of what is stored after the law
+        T hv = d.h;
+        bool flag = MIXED ? (int)__float_as_uint((float)d.h) < 0 : __double_as_longlong((double)d.h) < 0;
+        T s0 = rs0, sc = rsc, thr = rthr, c;
         if (PV == PV_TABLE) {
-            thr = tk[1];
-        } else if (PV == PV_GENERAL) {
+            // one 2-element LDS read: the record is 2 * sizeof(T) aligned
+            const T2<T> kt = *reinterpret_cast<const T2<T> *>(tk);
+            c = kt.x;
+            thr = kt.y;
+        } else {
+            c = tk[0];
+        }
+        if (PV == PV_GENERAL) {
             s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : rs0);
             sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : rsc);
             thr = law_threshold<LAW, T>(s0, sc, rs1);
@@ -813,22 +831,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         auto law = [&](T s) {
             if (!law_active<LAW, T>(s, thr, flag)) return;
             if (PV == PV_TABLE) {
-                const uint32_t ki = (uint32_t)key / KSCALE;
+                const uint32_t ki = (uint32_t)(key - tabl) / KSCALE;
                 s0 = bload<T>(rtab, (uint32_t)(ts0 + ki) * TW);
                 sc = bload<T>(rtab, (uint32_t)(tsc + ki) * TW);
             }
             uint32_t w = flag ? mark : 0u;
+            T H = MIXED ? (T)fabsf((float)d.h) : (T)fabs((double)d.h);
             apply_law<LAW, T>(s, s0, sc, rs1, lk, lalpha, ldenom, lsteel, lff, H, w);
             flag = (w & mark) != 0u;
-            const T hn = flag ? -H : H;
-            if (__builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, hn) !=
+            hv = flag ? -H : H;
+            if (__builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, hv) !=
                 __builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, d.h))
-                bstore<T>(hn, rh, lb, so);
+                bstore<T>(hv, rh, lb, so);
         };
         if constexpr (!MIXED) {
-            const double Xm0 = P.hx * (double)(kx + d.st.dx), Xm1 = P.hy * (double)(ky + d.st.dy),
-                         Xm2 = P.hz * (double)(kz + d.st.dz);
-            const double c = tk[0];
+            const double Xm0 = P.hx * (double)(kx + d.st.dx), Xm1 = P.hy * (double)(ky + d.st.dy()),
+                         Xm2 = P.hz * (double)(kz + d.st.dz());
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
             const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
@@ -841,7 +859,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
             else vcf = 0;
             const double fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0 : 1.0;
-            const double p = fail * c * (1 - H) * s * P.V * vcf;
+            const double p = fail * c * (1 - fabs(hv)) * s * P.V * vcf;
             // correctly rounded quotients dx/l etc. (see k_bond)
             const double rl = 1.0 / l;
             double qx = dx * rl, qy = dy * rl, qz = dz * rl;
@@ -856,23 +874,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             f1 += gy;
             f2 += gz;
         } else {
-            const float4 xi = make_float4(d.st.x, d.st.y, d.st.z, d.st.L0);
-            const float kap = tk[0] * d.st.vcf;
+            const float kap = c * d.st.vcf;
             const float et0 = (float)(nbx - qn0), et1 = (float)(nby - qn1), et2 = (float)(nbz - qn2);
-            const float y0 = xi.x + et0, y1 = xi.y + et1, y2 = xi.z + et2;
-            const float L0 = xi.w;
+            const float y0 = d.st.x + et0, y1 = d.st.y + et1, y2 = d.st.z + et2;
             // |y|^2 carries 1e-30 (below every representable bond's square by many orders, so it
             // changes no realistic length): 1/l stays finite for a bond of zero length, whose force
             // p * y is then 0, the reference's NaN -> 0 (calculatebondforcesMex.c:38)
             const float l2 = fmaf(y0, y0, fmaf(y1, y1, fmaf(y2, y2, 1e-30f)));
             const float rl = __builtin_amdgcn_rsqf(l2);
             const float l = l2 * rl;
-            const float num =
-                fmaf(2.0f, fmaf(xi.x, et0, fmaf(xi.y, et1, xi.z * et2)), fmaf(et0, et0, fmaf(et1, et1, et2 * et2)));
-            const float s = num * __builtin_amdgcn_rcpf(L0 * (l + L0));
+            // |y|^2 - |xi|^2 = eta . (2 xi + eta), and |xi| (l + |xi|) = |xi| l + |xi|^2
+            const float num = fmaf(et0, fmaf(2.0f, d.st.x, et0),
+                                   fmaf(et1, fmaf(2.0f, d.st.y, et1), et2 * fmaf(2.0f, d.st.z, et2)));
+            const float s = num * __builtin_amdgcn_rcpf(fmaf(d.st.L0, l, d.st.L0sq));
             law(s);
-            const float fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0f : 1.0f;
-            const float p = fail * kap * (1.0f - H) * s * rl;
+            // kap (1 - H) s / l; a failed bond of the brittle law carries no force
+            const float p = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0f : fmaf(-kap, fabsf(hv), kap) * (s * rl);
             a.x = fmaf(p, y0, a.x);
             a.y = fmaf(p, y1, a.y);
             a.z = fmaf(p, y2, a.z);
@@ -901,14 +918,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         load(A, 0);
         // the lane masks are wavefront-uniform: a lane's bit is tested by running the entry under
         // the mask as the execution mask
-        for (int o = 0; o < noff; o += 2) {
-            float3 a = make_float3(0.0f, 0.0f, 0.0f);
+        // offsets o, o + 1 with loads one offset ahead
+        auto pair = [&](int o, float3 &a) {
             load(B, o + 1);
             next_row(o);
             compute(A, o, __builtin_amdgcn_inverse_ballot_w64(A.mk), a);
             load(A, o + 2);
             next_row(o + 1);
             compute(B, o + 1, __builtin_amdgcn_inverse_ballot_w64(B.mk), a);
+        };
+        // groups of four offsets: the mixed form sums a group in FP32 and adds that to the FP64 row sum
+        for (int o = 0; o < noff; o += 4) {
+            float3 a = make_float3(0.0f, 0.0f, 0.0f);
+            pair(o, a);
+            if (o + 2 < noff) pair(o + 2, a);
             if constexpr (MIXED) {
                 f0 += (double)a.x;
                 f1 += (double)a.y;
@@ -1780,8 +1803,9 @@ std::vector<StOff> offset_records(const std::vector<int4> &off, const double *h,
         if (L0 <= hmr) vcf = 1;
         else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
         else vcf = 0;
-        r[o] = StOff{x0, x1, x2, std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2))), (float)vcf,
-                     off[o].y, off[o].z, off[o].w};
+        const float L0f = std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2)));
+        r[o] = StOff{x0, x1, x2, L0f, (float)vcf, (float)((double)L0f * (double)L0f), off[o].y,
+                     (off[o].z & 0xFFFF) | (int)((unsigned)off[o].w << 16)};
     }
     for (int k = 0; k < ST_PAD && !r.empty(); k++) r.push_back(r.back());
     return r;
