@@ -86,6 +86,9 @@ struct Region {
     int nrow, row0, rx;            // stencil layout: rows row0 .. row0 + nrow - 1, largest |dx|
     int sst0;                      // stencil layout: first per-offset record
     int kg, tab0, ntab, ts0, tsc;  // stencil layout: key tables (ts0 / tsc relative to tab0)
+    int fast64;                    // stencil layout, FP64 form: undeformed lengths, their reciprocals
+                                   // and the volume correction factors from the per-offset nominal
+                                   // values (checked bit for bit at set-up)
     double s0, s1, sc;             // scalar law parameters
     int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
     int schedule, nTimeSteps, ipi; // load schedule
@@ -122,9 +125,16 @@ struct __align__(32) NodeRec {
 
 // stencil layout, per offset (one 32-byte scalar load)
 struct __align__(32) StOff {
-    float x, y, z, L0; // mixed form: xi = h * d rounded to FP32, and |xi|
-    float vcf;         // mixed form: volume correction factor of |h * d|
-    float L0sq;        // mixed form: |xi|^2
+    union {
+        struct {
+            float x, y, z, L0; // mixed form: xi = h * d rounded to FP32, and |xi|
+            float vcf;         // mixed form: volume correction factor of |h * d|
+            float L0sq;        // mixed form: |xi|^2
+        };
+        struct {
+            double Ln, hn, rn; // FP64 form: a nominal length L of the offset, RN(0.5 / L), RN(1 / L)
+        };
+    };
     int dx;            // the lattice offset d: dx, and dy | dz << 16 (16-bit fields)
     int dyz;
     __host__ __device__ int dy() const { return (int)(int16_t)(dyz & 0xFFFF); }
@@ -158,6 +168,7 @@ struct Params {
     int law_linear; // the brittle latch: the stencil layout's sign bit is the failed flag
     // scalars shared by every region
     double V, horizon, hmr, hpr, two_r; // hmr = horizon - RADIJ, hpr = horizon + RADIJ, two_r = 2*RADIJ
+    double rtwo_r;                      // RN(1 / two_r)
     double k, alpha, denom_exp, sc_steel, ff;
     double DT, D;
     double *partials;     // per global block: the ADR Rayleigh partial sums
@@ -709,6 +720,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const Region &RG = P.reg[P.blkreg[gb]];
     const int base = RG.base, noff = RG.noff, off0 = RG.off0, kg = RG.kg, tab0 = RG.tab0, ntab = RG.ntab;
     const int ts0 = RG.ts0, tsc = RG.tsc;
+    const bool fast64 = !MIXED && RG.fast64;
     const bool real = n - base < RG.nN;
     const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = law_threshold<LAW, T>(rs0, rsc, rs1);
     const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
@@ -851,13 +863,33 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
             const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
-            const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
-            const double s = (l - L0) / L0;
+            const double q0 = (ex * ex + ey * ey) + ez * ez;
+            double L0, s;
+            if (fast64) {
+                // sqrt(q0) by one correction step and RN(1 / L0) by two Newton steps from the
+                // offset's nominal values (set-up checked both bit for bit for every value q0 takes
+                // on this lattice), and (l - L0) / L0 by Markstein's correction with RN(1 / L0)
+                L0 = fma(fma(-d.st.Ln, d.st.Ln, q0), d.st.hn, d.st.Ln);
+                const double r1 = fma(fma(-L0, d.st.rn, 1.0), d.st.rn, d.st.rn);
+                const double r = fma(fma(-L0, r1, 1.0), r1, r1);
+                const double a = l - L0, y = a * r;
+                s = fma(fma(-L0, y, a), r, y);
+            } else {
+                L0 = sqrt(q0);
+                s = (l - L0) / L0;
+            }
             law(s);
             double vcf; // calculatevolumecorrectionfactors.m:20-32
             if (L0 <= P.hmr) vcf = 1;
-            else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
-            else vcf = 0;
+            else if ((L0 > P.hmr) && (L0 <= P.horizon)) {
+                const double a = P.hpr - L0;
+                if (fast64) {
+                    const double y = a * P.rtwo_r;
+                    vcf = fma(fma(-P.two_r, y, a), P.rtwo_r, y);
+                } else {
+                    vcf = a / P.two_r;
+                }
+            } else vcf = 0;
             const double fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0 : 1.0;
             const double p = fail * c * (1 - fabs(hv)) * s * P.V * vcf;
             // correctly rounded quotients dx/l etc. (see k_bond)
@@ -1773,6 +1805,7 @@ struct StencilPlan {
     int nkey = 0;
     std::vector<double> tc, ts0, tsc; // key tables (empty when not tabled)
     int s0mode = PM_SCALAR, scmode = PM_SCALAR;
+    int kmin[3] = {0, 0, 0}, kmax[3] = {0, 0, 0}; // lattice coordinate ranges
 };
 
 // Lane masks of the stencil layout: per slice of 64 slots and per offset, bit lane set when the slot
@@ -1788,26 +1821,78 @@ std::vector<unsigned long long> lane_masks(const std::vector<uint4> &m, long lon
     return r;
 }
 
-// Per-offset records of the stencil layout: xi = h * d rounded to FP32 per component (an offset and
-// its negation give exactly negated vectors), |xi| from those, the volume correction factor of
-// |h * d| (calculatevolumecorrectionfactors.m:20-32), and d; then ST_PAD copies of the last.
+// Per-offset records of the stencil layout, then ST_PAD copies of the last.
+// Mixed form: xi = h * d rounded to FP32 per component (an offset and its negation give exactly
+// negated vectors), |xi| and |xi|^2 from those, and the volume correction factor of |h * d|
+// (calculatevolumecorrectionfactors.m:20-32).
+// FP64 form: a nominal length L of the offset with RN(0.5 / L) and RN(1 / L).  The bond kernel
+// rebuilds X = h * k per entry, so the undeformed length sqrt(q) (buildhorizons.m:93-97) of one
+// offset takes values up to some hundred ulps apart over the lattice; it takes sqrt(q) from the
+// nominal values by one correction step, RN(1 / L0) by two Newton steps, and vcf by Markstein's
+// correction with RN(1 / 2r).
+// `*fast64` says whether those steps give sqrt(q), RN(1 / L0) and the quotient (hpr - L0) / 2r bit
+// for bit for every value q takes: every combination of the per-axis differences
+// h k - h (k + d) over the coordinate ranges kmin .. kmax.
 std::vector<StOff> offset_records(const std::vector<int4> &off, const double *h, double hmr, double hpr,
-                                  double horizon, double two_r)
+                                  double horizon, double two_r, bool mixed, const int *kmin, const int *kmax,
+                                  int *fast64)
 {
     std::vector<StOff> r(off.size());
+    // per axis and offset component: the values h k - h (k + d) takes
+    auto diffs = [&](int ax, int dd) {
+        std::vector<double> v;
+        for (int k = kmin[ax]; k <= kmax[ax]; k++)
+            if (k + dd >= kmin[ax] && k + dd <= kmax[ax]) v.push_back(h[ax] * (double)k - h[ax] * (double)(k + dd));
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        return v;
+    };
+    const double rtwo_r = 1.0 / two_r;
+    bool ok = !mixed;
     for (size_t o = 0; o < off.size(); o++) {
+        StOff &R = r[o];
+        R.dx = off[o].y;
+        R.dyz = (off[o].z & 0xFFFF) | (int)((unsigned)off[o].w << 16);
         const double e0 = h[0] * off[o].y, e1 = h[1] * off[o].z, e2 = h[2] * off[o].w;
-        const float x0 = (float)e0, x1 = (float)e1, x2 = (float)e2;
-        const double L0 = std::sqrt((e0 * e0 + e1 * e1) + e2 * e2);
-        double vcf;
-        if (L0 <= hmr) vcf = 1;
-        else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
-        else vcf = 0;
-        const float L0f = std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2)));
-        r[o] = StOff{x0, x1, x2, L0f, (float)vcf, (float)((double)L0f * (double)L0f), off[o].y,
-                     (off[o].z & 0xFFFF) | (int)((unsigned)off[o].w << 16)};
+        if (mixed) {
+            const float x0 = (float)e0, x1 = (float)e1, x2 = (float)e2;
+            const double L0 = std::sqrt((e0 * e0 + e1 * e1) + e2 * e2);
+            double vcf;
+            if (L0 <= hmr) vcf = 1;
+            else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
+            else vcf = 0;
+            const float L0f = std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2)));
+            R.x = x0, R.y = x1, R.z = x2, R.L0 = L0f, R.vcf = (float)vcf;
+            R.L0sq = (float)((double)L0f * (double)L0f);
+            continue;
+        }
+        const std::vector<double> ex = diffs(0, off[o].y), ey = diffs(1, off[o].z), ez = diffs(2, off[o].w);
+        if (ex.empty() || ey.empty() || ez.empty()) {
+            // no bond has this offset on this lattice: any finite values
+            R.Ln = 1.0, R.hn = 0.5, R.rn = 1.0;
+            continue;
+        }
+        const double q = (ex[0] * ex[0] + ey[0] * ey[0]) + ez[0] * ez[0];
+        R.Ln = std::sqrt(q);
+        R.hn = 0.5 / R.Ln;
+        R.rn = 1.0 / R.Ln;
+        for (double a : ex)
+            for (double b : ey)
+                for (double c : ez) {
+                    const double qq = (a * a + b * b) + c * c;
+                    const double L0 = std::sqrt(qq);
+                    const double g = std::fma(std::fma(-R.Ln, R.Ln, qq), R.hn, R.Ln);
+                    const double r1 = std::fma(std::fma(-L0, R.rn, 1.0), R.rn, R.rn);
+                    const double rr = std::fma(std::fma(-L0, r1, 1.0), r1, r1);
+                    ok = ok && bits_equal(g, L0) && bits_equal(rr, 1.0 / L0);
+                    if ((L0 > hmr) && (L0 <= horizon)) {
+                        const double y = (hpr - L0) * rtwo_r;
+                        ok = ok && bits_equal(std::fma(std::fma(-two_r, y, hpr - L0), rtwo_r, y), (hpr - L0) / two_r);
+                    }
+                }
     }
     for (int k = 0; k < ST_PAD && !r.empty(); k++) r.push_back(r.back());
+    if (fast64) *fast64 = ok ? 1 : 0;
     return r;
 }
 
@@ -1833,7 +1918,8 @@ void plan_stencil(const bbpd_problem *pr, const double *X, const std::vector<int
     const int nN = pr->nNodes, nB = pr->nBonds;
     sp.ok = false;
     sp.k.assign(3 * (size_t)nN, 0);
-    int kmin[3] = {1 << 30, 1 << 30, 1 << 30}, kmax[3] = {-1, -1, -1};
+    int *kmin = sp.kmin, *kmax = sp.kmax;
+    for (int d = 0; d < 3; d++) kmin[d] = 1 << 30, kmax[d] = -1;
     for (int n = 0; n < nN; n++)
         for (int d = 0; d < 3; d++) {
             const int kk = (int)std::floor(X[n + (size_t)nN * d] / hlat[d] + 0.5);
@@ -2327,6 +2413,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     BBPD_HIP(hipMemcpy(rowlend, rowlen.data(), sizeof(int) * nSlots, hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy(soffd, slice_off.data(), sizeof(long long) * slice_off.size(), hipMemcpyHostToDevice));
     if (!sp.ok) BBPD_HIP(hipMemcpy(recd, h_rec.data(), sizeof(uint4) * nE, hipMemcpyHostToDevice));
+    int fast64 = 0; // FP64 stencil kernel: per-offset nominal lengths apply (offset_records)
     if (sp.ok) {
         // per-slot masks; history 0, its sign bit set for bonds that start failed (brittle law)
         std::vector<uint4> hm(nSlots, make_uint4(0, 0, 0, 0));
@@ -2356,7 +2443,12 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         BBPD_HIP(hipMemcpy(srowd, sp.rows.data(), sizeof(int2) * sp.rows.size(), hipMemcpyHostToDevice));
         // mixed form per offset: xi = h * d rounded to FP32 (so the offset and its negation give
         // exactly negated vectors), |xi|, and the volume correction factor of |h * d|
-        const std::vector<StOff> hst = offset_records(sp.off, hlat, hmr, hpr, horizon, two_r);
+        const std::vector<StOff> hst =
+            offset_records(sp.off, hlat, hmr, hpr, horizon, two_r, mixed, sp.kmin, sp.kmax, &fast64);
+        // BBPD_FAST64=0 asks for the lengths computed per entry
+        const char *f64 = std::getenv("BBPD_FAST64");
+        if (f64 && std::string(f64) == "0") fast64 = 0;
+        if (!mixed && !fast64) s->layout = "stencil (FP64 lengths computed per entry)";
         BBPD_HIP(hipMemcpy(sstd, hst.data(), sizeof(StOff) * hst.size(), hipMemcpyHostToDevice));
         if (mixed) {
             // c * V, then the parameter tables, in FP32
@@ -2420,6 +2512,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.hmr = hmr;
     P.hpr = hpr;
     P.two_r = two_r;
+    P.rtwo_r = 1.0 / two_r;
     P.k = pr->k;
     P.alpha = pr->alpha;
     P.denom_exp = 1 - std::exp(-pr->k);
@@ -2458,6 +2551,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.row0 = 0;
     R0.rx = sp.rx;
     R0.sst0 = 0;
+    R0.fast64 = fast64;
     R0.kg = sp.G;
     R0.tab0 = 0;
     R0.ntab = (int)(tabc.size() + tabp.size());
@@ -3400,6 +3494,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     P.hmr = sh->horizon - sh->RADIJ;
     P.hpr = sh->horizon + sh->RADIJ;
     P.two_r = 2 * sh->RADIJ;
+    P.rtwo_r = 1.0 / P.two_r;
     P.k = sh->k;
     P.alpha = sh->alpha;
     P.denom_exp = 1 - std::exp(-sh->k);
@@ -3591,7 +3686,9 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     // per-offset data and key tables
     const double hmr = sh.horizon - sh.RADIJ, hpr = sh.horizon + sh.RADIJ, two_r = 2 * sh.RADIJ;
     const double hdx[3] = {s->DX, s->DX, s->DX};
-    const std::vector<StOff> hst = offset_records(sp.off, hdx, hmr, hpr, sh.horizon, two_r);
+    int fast64 = 0;
+    const std::vector<StOff> hst =
+        offset_records(sp.off, hdx, hmr, hpr, sh.horizon, two_r, mixed, sp.kmin, sp.kmax, &fast64);
     std::vector<double> tab(sp.tc);
     tab.insert(tab.end(), sp.ts0.begin(), sp.ts0.end());
     tab.insert(tab.end(), sp.tsc.begin(), sp.tsc.end());
@@ -3614,6 +3711,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.row0 = rid * (ST_MAX + ST_PAD);
     RG.rx = sp.rx;
     RG.sst0 = rid * (ST_MAX + ST_PAD);
+    RG.fast64 = fast64;
     RG.kg = sp.G;
     RG.tab0 = rid * RB_TAB;
     RG.ntab = (int)tab.size();

@@ -80,7 +80,7 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=(), retune=False):
     if warmup in keep:
         ckpts[warmup] = sol.checkpoint()
     tune = None
-    tunable = case.integrator == _lib.ADR and sol.layout() == "stencil"
+    tunable = case.integrator == _lib.ADR and sol.layout().startswith("stencil")
     if tunable:
         tune = sol.autotune(5)
         log(f"  autotune: variant {tune[0]} of {['%.3f' % x for x in tune[1]]} ms")

@@ -251,7 +251,11 @@ int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel
 /* the bond-table layout chosen at set-up: "stencil" (full lattice box, rows in lattice-offset
  * order: neighbours implicit, per entry only the history) or "records (<why not stencil>)".  The
  * string lives as long as the solver.  Default: stencil where it applies, for both forms;
- * BBPD_LAYOUT=records in the environment asks for the record layout. */
+ * BBPD_LAYOUT=records in the environment asks for the record layout.  The FP64 stencil kernel takes
+ * the undeformed lengths, their reciprocals and the volume correction factors from per-offset
+ * nominal values where set-up finds that exact for every value they take on the lattice, else
+ * computes them per entry and reports "stencil (FP64 lengths computed per entry)"; BBPD_FAST64=0
+ * asks for the latter. */
 const char *bbpd_solver_layout(bbpd_solver *s);
 /* Autotuner: times each compiled launch variant of the stencil bond kernel over `reps` launches on
  * the current state and keeps the fastest for the solver or batch.  Variants: 0 = compiler's

@@ -357,6 +357,45 @@ def test_stencil_layout_wide_rows(gpu, horizon_factor):
         assert np.abs(a.nodal()[0]).max() > 0
 
 
+def _solver_with_env(case, precision, **env):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    try:
+        os.environ.update(env)
+        return Solver(case, precision)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("dx", [5e-3, 1.25e-3])
+def test_fp64_nominal_lengths_equal_per_entry(gpu, dx):
+    """FP64 stencil kernel: the undeformed lengths, their reciprocals and the volume correction
+    factors taken from per-offset nominal values (one correction step each, checked at set-up on
+    every value they take on the lattice) give the ADR trajectory of computing them per entry with
+    square root and division, bit for bit, on an exponential-law beam at two lattice spacings."""
+    case = beam.build_beam("t", 24, 6, 10, dx, law=_lib.LAW_EXPONENTIAL, s0=1e-4, integrator=_lib.ADR,
+                           iters_per_increment=100, increment=-4e-3 * dx,
+                           notch=(12.5, 3.5))
+    print(f"case: 24 x 6 x 10 beam, dx {dx}, {case.nBonds} bonds", flush=True)
+    a = _solver_with_env(case, _lib.FP64)
+    b = _solver_with_env(case, _lib.FP64, BBPD_FAST64="0")
+    with a, b:
+        assert a.layout() == "stencil", a.layout()
+        assert b.layout() == "stencil (FP64 lengths computed per entry)", b.layout()
+        a.step(600)
+        b.step(600)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+        assert same(a.history()["reaction"], b.history()["reaction"])
+        assert a.observables()["n_softening"] > 0
+
+
 def test_stencil_layout_restore_and_adr(gpu):
     """A checkpoint of the stencil layout restores into the record layout (the history's sign-bit
     flag round-trips) and the two continue bit for bit under Euler-Cromer.  Under ADR the damping
