@@ -43,6 +43,27 @@ __device__ inline double block_sum_fixed(double v, double *sh)
     return r;
 }
 
+// Two block sums in one pass of the same fixed tree (each equals block_sum_fixed of its value);
+// sh holds 2 * BLOCK doubles.
+template <int BLOCK>
+__device__ inline void block_sum_fixed2(double &a, double &b, double *sh)
+{
+    sh[threadIdx.x] = a;
+    sh[BLOCK + threadIdx.x] = b;
+    __syncthreads();
+#pragma unroll
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+            sh[BLOCK + threadIdx.x] = sh[BLOCK + threadIdx.x] + sh[BLOCK + threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    a = sh[0];
+    b = sh[BLOCK];
+    __syncthreads();
+}
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline unsigned grid_for(long n, int block)

@@ -981,16 +981,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
 // (calculatedampingcoefficient.m:90-112); one block per region, a fixed summation order
 __global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P)
 {
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[2 * BLOCK];
     const int ri = P.active[blockIdx.x];
     const Region &RG = P.reg[ri];
     double c1 = 0, c2 = 0;
+    // the loads of eight steps are issued together; each thread still adds its partials in order
+    const double2 *part = reinterpret_cast<const double2 *>(P.partials) + RG.blk0;
+#pragma unroll 8
     for (int q = threadIdx.x; q < RG.nblk; q += BLOCK) {
-        c1 = c1 + P.partials[2 * (RG.blk0 + q)];
-        c2 = c2 + P.partials[2 * (RG.blk0 + q) + 1];
+        const double2 p = part[q];
+        c1 = c1 + p.x;
+        c2 = c2 + p.y;
     }
-    const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
-    const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
+    bbpd::block_sum_fixed2<BLOCK>(c1, c2, sh);
+    const double s1 = c1, s2 = c2;
     if (threadIdx.x == 0) {
         double cn = 0;
         if (s2 != 0) cn = ((s1 / s2) > 0) ? 2 * sqrt(s1 / s2) : 0;
