@@ -12,7 +12,7 @@ Reported (one JSON line on stdout):
   precision        FP32-mixed against FP64 on the load history: peak of the increment-averaged load
                    curve, the FP64 run's own rounding sensitivity beside it, and per-result errors
                    (load, nodal force, flags, broken bonds, damage field) from identical FP64 states
-  roofline         dominant kernel (bond kernel, mixed form): the byte model of the hot-path survey,
+  roofline         dominant kernel (bond kernel, mixed form): the bytes of a node-centric CSR layout,
                    E*(4 + w_state + w_kappa + n_par*w) + N*180 with E = 2*nBonds entries, divided by
                    the kernel's average launch time from HIP events on the solver's stream, against
                    8 TB/s (a fraction above 1 means the stencil layout moves fewer bytes than that
@@ -48,7 +48,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def survey_model_bytes(case, mixed: bool) -> int:
+def csr_model_bytes(case, mixed: bool) -> int:
     """Bytes one bond-kernel launch must move in the node-centric layout with kappa stored per
     entry: E*(4 + w_state + w_kappa + n_par*w) + N*180."""
     w = 4 if mixed else 8
@@ -110,7 +110,7 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=(), retune=False):
     info = sol.info()
     hist = {k: np.concatenate([r[k] for r in rows]) for k in rows[0]}
     mixed = precision == _lib.MIXED
-    model = survey_model_bytes(case, mixed)
+    model = csr_model_bytes(case, mixed)
     achieved = model / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return dict(solver=sol, seconds=dt, ms_per_step=dt * 1e3 / steps, bond_updates_per_s=case.nBonds * steps / dt,
                 kernel_ms=avg_ms, launches=launches, bytes_model=model, bytes_layout=info["bond_kernel_bytes"],
