@@ -1081,8 +1081,9 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
 {
     // the last block of a region records the region's observables and advances its step counter;
     // the last block of the launch advances the position-buffer parity
-    __shared__ double vals[1024];
-    __shared__ int wofs[17];
+    constexpr int SE_CH = 4; // family chunks of blockDim.x members per group
+    __shared__ double vals[SE_CH * 1024];
+    __shared__ int wofs[SE_CH * 16 + 1]; // per (chunk, wavefront) contact offsets, then the count
     __shared__ int last_block, last_all;
     const int2 job = P.se[blockIdx.x];
     const int ri = job.x, bi = job.y;
@@ -1105,7 +1106,6 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
         // the members in groups of SE_CH chunks of blockDim.x: every load of a group is issued
         // before any of its members is projected, so a family costs about one memory latency per
         // group rather than one chain of dependent loads per chunk
-        constexpr int SE_CH = 4;
         for (int g0 = 0; g0 < nf; g0 += SE_CH * (int)blockDim.x) {
             int idx[SE_CH];
             NodeRec pr[SE_CH];
@@ -1166,23 +1166,38 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
                     pos[i].z = pv[2];
                 }
             }
-            // compact the members in contact, in family order, then sum them in that order
-            // (the order of MATLAB's sum over dpenfz; members not in contact add nothing)
-            for (int k = 0; k < SE_CH && g0 + k * (int)blockDim.x < nf; k++) {
-                const unsigned long long mask = __ballot(hit[k]);
-                if (lane == 0) wofs[wv + 1] = __popcll(mask);
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    wofs[0] = 0;
-                    for (int q = 1; q <= nw; q++) wofs[q] += wofs[q - 1];
-                }
-                __syncthreads();
-                if (hit[k]) vals[wofs[wv] + __popcll(mask & ((1ull << lane) - 1))] = d_z[k];
-                __syncthreads();
-                if (threadIdx.x == 0)
-                    for (int q = 0; q < wofs[nw]; q++) total = total + vals[q];
-                __syncthreads();
+            // compact the group's members in contact, in family order (chunk, wavefront, lane), then
+            // sum them in that order (the order of MATLAB's sum over dpenfz; members not in contact
+            // add nothing).  The SE_CH * nw <= 64 counts are scanned by one wavefront.
+            unsigned long long mask[SE_CH];
+#pragma unroll
+            for (int k = 0; k < SE_CH; k++) {
+                mask[k] = __ballot(hit[k]);
+                if (lane == 0) wofs[k * nw + wv] = __popcll(mask[k]);
             }
+            __syncthreads();
+            if (wv == 0) {
+                const int c = lane < SE_CH * nw ? wofs[lane] : 0;
+                int incl = c;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(incl, d);
+                    if (lane >= d) incl += t;
+                }
+                if (lane < SE_CH * nw) wofs[lane] = incl - c; // exclusive offsets
+                if (lane == 63) wofs[SE_CH * 16] = incl;     // the group's contacts
+            }
+            __syncthreads();
+            const int ncon = wofs[SE_CH * 16];
+#pragma unroll
+            for (int k = 0; k < SE_CH; k++)
+                if (hit[k]) vals[wofs[k * nw + wv] + __popcll(mask[k] & ((1ull << lane) - 1))] = d_z[k];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+#pragma unroll 8
+                for (int q = 0; q < ncon; q++) total = total + vals[q];
+            }
+            __syncthreads();
         }
         // the static reaction: the threads' shares summed in a fixed tree
         if (bi == 0) rs = bbpd::block_sum_fixed<1024>(part, vals);
