@@ -15,6 +15,7 @@ LAW_LINEAR, LAW_BILINEAR, LAW_TRILINEAR, LAW_EXPONENTIAL = 0, 1, 2, 3
 EULER_CROMER, ADR = 0, 1
 FP64, MIXED = 0, 1
 MAX_BODIES = 8
+AUTOTUNE_VARIANTS = 6  # BBPD_AUTOTUNE_VARIANTS
 
 _dp = C.POINTER(C.c_double)
 

@@ -126,7 +126,7 @@ class Batch:
     def autotune(self, reps: int = 5):
         """Time the bond-kernel variants on the current state (ADR only), keep the fastest; returns
         (index kept, average ms per variant)."""
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * _lib.AUTOTUNE_VARIANTS)()
         ch = C.c_int()
         _lib.check(self._L.bbpd_solver_autotune(self._h, reps, ms, C.byref(ch)), "bbpd_solver_autotune")
         return ch.value, list(ms)

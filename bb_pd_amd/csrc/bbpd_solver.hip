@@ -1534,6 +1534,8 @@ struct bbpd_solver {
     Region *d_reg = nullptr;
     RegionState *d_rst = nullptr;
     int *d_blkreg = nullptr, *d_grid = nullptr, *d_active = nullptr;
+    int *d_grid_st = nullptr; // the launch list in XCD stripes (launch variants 4 and 5)
+    int grid_n_st = 0;
     int2 *d_se = nullptr;
     Body *d_bodies = nullptr;
     long long *d_tick = nullptr;
@@ -1566,7 +1568,7 @@ struct bbpd_solver {
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     int graph_version = -1;
-    int occ = 0; // occupancy variant of the stencil bond kernel (index into OCC_WAVES)
+    int occ = 0; // launch variant of the stencil bond kernel (pick_occ)
 };
 
 namespace {
@@ -1590,8 +1592,10 @@ KernelFn pick_law(int law, bool lat)
     }
 }
 
-// launch variants of the stencil bond kernel: (minimum waves per SIMD, XCD block remap)
-constexpr int N_OCC = 4;
+// launch variants of the stencil bond kernel: (minimum waves per SIMD, block order); orders: XCD
+// remap, launch order, and launch order over the stripe list (variants 4 and 5)
+constexpr int N_OCC = BBPD_AUTOTUNE_VARIANTS;
+constexpr int ST_STRIPE = 8; // consecutive global blocks per XCD in the stripe list
 
 template <bool MIXED, int LAW, int PV>
 KernelFn pick_occ(int v)
@@ -1600,7 +1604,36 @@ KernelFn pick_occ(int v)
     case 1: return k_bond_st<MIXED, LAW, 1, false, PV>;
     case 2: return k_bond_st<MIXED, LAW, 5, true, PV>;
     case 3: return k_bond_st<MIXED, LAW, 5, false, PV>;
+    case 4: return k_bond_st<MIXED, LAW, 1, false, PV>;
+    case 5: return k_bond_st<MIXED, LAW, 5, false, PV>;
     default: return k_bond_st<MIXED, LAW, 1, true, PV>;
+    }
+}
+
+// The launch list in XCD stripes: launch block b runs on XCD b % 8 (the hardware's placement), and
+// the b-th launch takes entry j % S of stripe b % 8 of group j / S (j = b / 8, S = ST_STRIPE) of the
+// list, so each XCD walks runs of S consecutive list entries (neighbour rows shared in its L2) while
+// a localised process zone still spreads over all XCDs.  -1 = idle.
+std::vector<int> stripe_list(const std::vector<int> &list)
+{
+    const int L = (int)list.size(), M = 8 * ST_STRIPE;
+    std::vector<int> r((size_t)((L + M - 1) / M) * M, -1);
+    for (int b = 0; b < (int)r.size(); b++) {
+        const int j = b >> 3, g = (j / ST_STRIPE) * M + (b & 7) * ST_STRIPE + j % ST_STRIPE;
+        r[b] = g < L ? list[g] : -1;
+    }
+    return r;
+}
+
+// Launch the solver's bond kernel `kb` of launch variant v
+void launch_bond(bbpd_solver *s, KernelFn kb, int v)
+{
+    if (v >= 4) {
+        Params Pb = s->P;
+        Pb.grid = s->d_grid_st;
+        hipLaunchKernelGGL(kb, dim3(s->grid_n_st), dim3(BLOCK), s->lds_bytes, s->stream, Pb);
+    } else {
+        hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
     }
 }
 
@@ -1668,7 +1701,7 @@ int launch_step(bbpd_solver *s, bool allow_timing)
         }
         BBPD_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     }
-    hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
+    launch_bond(s, kb, s->stencil ? s->occ : 0);
     BBPD_HIP(hipGetLastError());
     if (timed) {
         BBPD_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
@@ -2649,6 +2682,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         if ((rc = dalloc(&s->d_rst, 1, s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_blkreg, nblk, s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_grid, nblk, s->owned))) return fail(rc);
+        const std::vector<int> gst = stripe_list(grid);
+        if ((rc = dalloc(&s->d_grid_st, gst.size(), s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_active, 1, s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_se, se.size(), s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_bodies, hb.size(), s->owned))) return fail(rc);
@@ -2661,6 +2696,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         BBPD_HIP(hipMemcpy(s->d_rst, &z, sizeof(RegionState), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_blkreg, blkreg.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_grid, grid.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_grid_st, gst.data(), sizeof(int) * gst.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_active, &act, sizeof(int), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int2) * se.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_bodies, hb.data(), sizeof(Body) * hb.size(), hipMemcpyHostToDevice));
@@ -2670,6 +2706,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         s->active.assign(1, 1);
         s->hist_read.assign(1, 0);
         s->grid_n = nblk;
+        s->grid_n_st = (int)gst.size();
         s->nse = (int)se.size();
         s->nactive = 1;
         P.reg = s->d_reg;
@@ -3258,11 +3295,10 @@ extern "C" int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *c
     int bestv = s->occ;
     for (int v = 0; v < N_OCC; v++) {
         KernelFn kb = pick_bond_kernel(s, v);
-        hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P); // warm-up
+        launch_bond(s, kb, v); // warm-up
         BBPD_HIP(hipGetLastError());
         BBPD_HIP(hipEventRecord(e0, s->stream));
-        for (int r = 0; r < reps; r++)
-            hipLaunchKernelGGL(kb, dim3(s->grid_n), dim3(BLOCK), s->lds_bytes, s->stream, s->P);
+        for (int r = 0; r < reps; r++) launch_bond(s, kb, v);
         BBPD_HIP(hipEventRecord(e1, s->stream));
         BBPD_HIP(hipEventSynchronize(e1));
         float t = 0;
@@ -3353,9 +3389,12 @@ int batch_rebuild(bbpd_solver *s)
     BBPD_HIP(hipStreamSynchronize(s->stream));
     BBPD_HIP(hipMemcpy(s->d_blkreg, blkreg.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
     if (!grid.empty()) BBPD_HIP(hipMemcpy(s->d_grid, grid.data(), sizeof(int) * grid.size(), hipMemcpyHostToDevice));
+    const std::vector<int> gst = stripe_list(grid);
+    if (!gst.empty()) BBPD_HIP(hipMemcpy(s->d_grid_st, gst.data(), sizeof(int) * gst.size(), hipMemcpyHostToDevice));
     if (!se.empty()) BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int2) * se.size(), hipMemcpyHostToDevice));
     if (!act.empty()) BBPD_HIP(hipMemcpy(s->d_active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice));
     s->grid_n = (int)grid.size();
+    s->grid_n_st = (int)gst.size();
     s->nse = (int)se.size();
     s->nactive = (int)act.size();
     s->P.nse = s->nse;
@@ -3457,6 +3496,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if ((rc = dalloc(&s->d_rst, s->maxRegions, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_blkreg, nblk, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_grid, nblk + 8, s->owned))) return fail(rc);
+    if ((rc = dalloc(&s->d_grid_st, nblk + 8 + 8 * ST_STRIPE, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_active, s->maxRegions, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_se, (size_t)s->maxRegions * BBPD_MAX_BODIES, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_bodies, (size_t)s->maxRegions * BBPD_MAX_BODIES, s->owned))) return fail(rc);

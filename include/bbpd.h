@@ -260,11 +260,14 @@ const char *bbpd_solver_layout(bbpd_solver *s);
 /* Autotuner: times each compiled launch variant of the stencil bond kernel over `reps` launches on
  * the current state and keeps the fastest for the solver or batch.  Variants: 0 = compiler's
  * register budget, XCD-contiguous block order; 1 = the same, blocks in launch order; 2 / 3 = at
- * least 5 waves per SIMD (fewer registers per wave), XCD order / launch order.  XCD order keeps a
- * block's neighbour rows in its XCD's L2; launch order spreads a localised process zone over all
- * XCDs.  ms (may be NULL) receives the 4
- * averages, choice the index kept.  Needs the ADR integrator (whose bond kernel leaves the state
- * unchanged when rerun); a no-op on the record layout.  No counterpart in the reference. */
+ * least 5 waves per SIMD (fewer registers per wave), XCD order / launch order; 4 / 5 = the
+ * register budgets of 0 / 2 with the blocks in XCD stripes (each XCD takes runs of 8 consecutive
+ * blocks, the runs dealt out in launch order).  XCD order keeps a block's neighbour rows in its
+ * XCD's L2; launch order spreads a localised process zone over all XCDs; stripes do some of both.
+ * ms (may be NULL) receives BBPD_AUTOTUNE_VARIANTS averages, choice the index kept.  Needs the ADR
+ * integrator (whose bond kernel leaves the state unchanged when rerun); a no-op on the record
+ * layout.  No counterpart in the reference. */
+#define BBPD_AUTOTUNE_VARIANTS 6
 int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *choice);
 /* average duration (ms) of the bond kernel over the steps since the last call, from HIP events
  * recorded on the solver's stream around each launch (0 when timing is off) */
