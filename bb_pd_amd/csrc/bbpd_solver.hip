@@ -812,7 +812,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     };
     // One entry.  The FP64 form adds its force to the row sum here; the mixed form adds it to the
     // FP32 sum `a` of a group of up to four entries, which the loop adds to the FP64 row sum.
+    // FP64 form: the y and z terms of the undeformed length's square, (Y_n - Y_m)^2 and (Z_n - Z_m)^2,
+    // depend on the offset's (dy, dz) only, which a stencil row shares: computed once per row, for
+    // every lane (before the lane-mask test)
+    int cyz = INT_MIN;
+    double eyy = 0, ezz = 0;
     auto compute = [&](const StEnt<T> &d, int o, bool bit, float3 &a) {
+        if (!MIXED && d.st.dyz != cyz) {
+            cyz = d.st.dyz;
+            const double ey = Xn1 - P.hy * (double)(ky + d.st.dy()), ez = Xn2 - P.hz * (double)(kz + d.st.dz());
+            eyy = ey * ey;
+            ezz = ez * ez;
+        }
         if (!bit) return;
         const uint32_t so = (uint32_t)o * (SLICE * TW);
         const double *nb = &s_row[wv][0] + (lane + rx + d.st.dx);
@@ -857,13 +868,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
                 bstore<T>(hv, rh, lb, so);
         };
         if constexpr (!MIXED) {
-            const double Xm0 = P.hx * (double)(kx + d.st.dx), Xm1 = P.hy * (double)(ky + d.st.dy()),
-                         Xm2 = P.hz * (double)(kz + d.st.dz());
+            const double Xm0 = P.hx * (double)(kx + d.st.dx);
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
             const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
-            const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
-            const double q0 = (ex * ex + ey * ey) + ez * ez;
+            // buildhorizons.m:93-97: (ex^2 + ey^2) + ez^2, the y and z squares from the row
+            const double ex = Xn0 - Xm0;
+            const double q0 = (ex * ex + eyy) + ezz;
             double L0, s;
             if (fast64) {
                 // sqrt(q0) by one correction step and RN(1 / L0) by two Newton steps from the
