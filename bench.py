@@ -94,7 +94,7 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=(), retune=False):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if retune and tunable and done > 0:
-            picks.append(sol.autotune(3)[0])
+            picks.append(sol.autotune(2)[0])
         sol.step(n)
         sol.sync()
         torch.cuda.synchronize()

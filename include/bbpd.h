@@ -264,9 +264,10 @@ const char *bbpd_solver_layout(bbpd_solver *s);
  * register budgets of 0 / 2 with the blocks in XCD stripes (each XCD takes runs of 8 consecutive
  * blocks, the runs dealt out in launch order).  XCD order keeps a block's neighbour rows in its
  * XCD's L2; launch order spreads a localised process zone over all XCDs; stripes do some of both.
- * ms (may be NULL) receives BBPD_AUTOTUNE_VARIANTS averages, choice the index kept.  Needs the ADR
- * integrator (whose bond kernel leaves the state unchanged when rerun); a no-op on the record
- * layout.  No counterpart in the reference. */
+ * The first call runs each variant once untimed before timing it (loading its code); later calls
+ * time at once.  ms (may be NULL) receives BBPD_AUTOTUNE_VARIANTS averages, choice the index kept.
+ * Needs the ADR integrator (whose bond kernel leaves the state unchanged when rerun); a no-op on the
+ * record layout.  No counterpart in the reference. */
 #define BBPD_AUTOTUNE_VARIANTS 6
 int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *choice);
 /* average duration (ms) of the bond kernel over the steps since the last call, from HIP events
