@@ -814,11 +814,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     // FP32 sum `a` of a group of up to four entries, which the loop adds to the FP64 row sum.
     // FP64 form: the y and z terms of the undeformed length's square, (Y_n - Y_m)^2 and (Z_n - Z_m)^2,
     // depend on the offset's (dy, dz) only, which a stencil row shares: computed once per row, for
-    // every lane (before the lane-mask test)
+    // every lane (before the lane-mask test).  Not in the variant with streamed parameters, where the
+    // two registers they hold across the row cost more in spills than the row saves.
     int cyz = INT_MIN;
     double eyy = 0, ezz = 0;
     auto compute = [&](const StEnt<T> &d, int o, bool bit, float3 &a) {
-        if (!MIXED && d.st.dyz != cyz) {
+        if (!MIXED && PV != PV_GENERAL && d.st.dyz != cyz) {
             cyz = d.st.dyz;
             const double ey = Xn1 - P.hy * (double)(ky + d.st.dy()), ez = Xn2 - P.hz * (double)(kz + d.st.dz());
             eyy = ey * ey;
@@ -874,7 +875,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
             // buildhorizons.m:93-97: (ex^2 + ey^2) + ez^2, the y and z squares from the row
             const double ex = Xn0 - Xm0;
-            const double q0 = (ex * ex + eyy) + ezz;
+            double q0;
+            if (PV == PV_GENERAL) {
+                const double ey = Xn1 - P.hy * (double)(ky + d.st.dy()), ez = Xn2 - P.hz * (double)(kz + d.st.dz());
+                q0 = (ex * ex + ey * ey) + ez * ez;
+            } else {
+                q0 = (ex * ex + eyy) + ezz;
+            }
             double L0, s;
             if (fast64) {
                 // sqrt(q0) by one correction step and RN(1 / L0) by two Newton steps from the
