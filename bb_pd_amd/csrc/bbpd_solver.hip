@@ -674,6 +674,35 @@ __device__ __forceinline__ NodeRec bload_rec(brsrc r, uint32_t voff)
     return __builtin_bit_cast(NodeRec, w);
 }
 
+// FP64 sqrt(x) and 1 / x for x in [2^-767, 2^1000]: the same operations as the compiler's IEEE
+// sequences for these (reciprocal square root with Goldschmidt iterations; reciprocal with two Newton
+// steps and the FMA correction of the quotient), without the operand scaling and special-case
+// handling those carry for exponents near the limits and for 0, infinity and NaN, which take no effect
+// in this range: the same bits.
+constexpr double CORE_LO = 0x1p-767, CORE_HI = 0x1p+1000;
+__device__ __forceinline__ double sqrt_core(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    double d = fma(-g, g, x);
+    h = fma(h, e, h);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+__device__ __forceinline__ double rcp_core(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(e, r, r);
+}
+
 // The law's lowest threshold.  Every value a law stores lies in [0, 1]
 // (calculatebsfexponentialMex.c:55-58 clamps; the others are maxima of values in [0, 1]) and set-up
 // and restore accept only such values; with BONDTYPE 0 (the stencil layout's only type), a stretch
@@ -870,9 +899,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         };
         if constexpr (!MIXED) {
             const double Xm0 = P.hx * (double)(kx + d.st.dx);
-            // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n)
+            // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n); l and 1 / l by the unscaled
+            // sequences when every lane's squared length lies in their range (wave-uniform branch)
             const double dx = nbx - qn0, dy = nby - qn1, dz = nbz - qn2;
-            const double l = sqrt(dx * dx + dy * dy + dz * dz);
+            const double l2 = dx * dx + dy * dy + dz * dz;
+            const bool core = __builtin_amdgcn_ballot_w64(!(l2 >= CORE_LO && l2 <= CORE_HI)) == 0;
+            double l, rl;
+            if (core) {
+                l = sqrt_core(l2);
+                rl = rcp_core(l);
+            } else {
+                l = sqrt(l2);
+                rl = 1.0 / l;
+            }
             // buildhorizons.m:93-97: (ex^2 + ey^2) + ez^2, the y and z squares from the row
             const double ex = Xn0 - Xm0;
             double q0;
@@ -911,7 +950,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             const double fail = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0 : 1.0;
             const double p = fail * c * (1 - fabs(hv)) * s * P.V * vcf;
             // correctly rounded quotients dx/l etc. (see k_bond)
-            const double rl = 1.0 / l;
             double qx = dx * rl, qy = dy * rl, qz = dz * rl;
             qx = fma(fma(-l, qx, dx), rl, qx);
             qy = fma(fma(-l, qy, dy), rl, qy);
