@@ -396,6 +396,37 @@ def test_fp64_nominal_lengths_equal_per_entry(gpu, dx):
         assert a.observables()["n_softening"] > 0
 
 
+def test_stencil_zero_length_bond_equals_records(gpu):
+    """Two neighbouring nodes moved onto one another: their bond has length 0, whose force the
+    reference turns from NaN into 0 (calculatebondforcesMex.c:38-40).  The FP64 stencil kernel takes
+    the deformed length and its reciprocal from the unscaled square-root and reciprocal sequences only
+    when every lane of the wavefront is in their range, so this wavefront takes the library's IEEE
+    sequences; the two layouts stay bit-identical and every force stays finite."""
+    case = ec_case(_lib.LAW_BILINEAR)
+    X = case.X
+    DX = X[:, 0].min()
+    n = int(np.flatnonzero(np.isclose(X[:, 0], 10 * DX) & np.isclose(X[:, 1], 3 * DX) & np.isclose(X[:, 2], 5 * DX))[0])
+    m = int(np.flatnonzero(np.isclose(X[:, 0], 11 * DX) & np.isclose(X[:, 1], 3 * DX) & np.isclose(X[:, 2], 5 * DX))[0])
+    a = _solver_with_layout(case, _lib.FP64, "stencil")
+    b = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b:
+        assert a.layout() == "stencil", a.layout()
+        a.step(20)
+        st = a.checkpoint()
+        step = st.pop("step")
+        st["x"][n] = st["x"][m]
+        st["u"][n] = st["x"][n] - X[n]
+        a.restore(step, **st)
+        b.restore(step, **st)
+        a.step(3)
+        b.step(3)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+            assert np.isfinite(x).all()
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+
+
 def test_stencil_layout_restore_and_adr(gpu):
     """A checkpoint of the stencil layout restores into the record layout (the history's sign-bit
     flag round-trips) and the two continue bit for bit under Euler-Cromer.  Under ADR the damping
