@@ -92,3 +92,15 @@ def test_beam_geometry():
     assert d.min() == pytest.approx(pen.R, rel=1e-12)
     assert s1.cz == pytest.approx(-s1.R + c.DX) and s1.cx == pytest.approx(0.025)
     assert pen.moves and not s1.moves and not s2.moves
+
+
+def test_smoke_case_has_disjoint_contact_families():
+    """The fused solver rejects overlapping contact families (a node projected by two bodies in one
+    step); the smoke run's beam must not have any."""
+    import __graft_entry__ as G
+    case = G.smoke_case()
+    fams = [set(np.asarray(b.family).astype(int).tolist()) for b in case.bodies]
+    assert len(fams) == 3 and all(fams)
+    for i in range(len(fams)):
+        for j in range(i + 1, len(fams)):
+            assert not (fams[i] & fams[j]), (i, j)
