@@ -1682,6 +1682,16 @@ std::vector<int> stripe_list(const std::vector<int> &list)
     return r;
 }
 
+// BBPD_LAUNCH_VARIANT=v in the environment fixes the stencil bond kernel's launch variant at set-up
+// (the autotuner may change it later); -1 = unset
+int env_launch_variant()
+{
+    const char *e = std::getenv("BBPD_LAUNCH_VARIANT");
+    if (!e || !*e) return -1;
+    const int v = std::atoi(e);
+    return v >= 0 && v < N_OCC ? v : -1;
+}
+
 // Launch the solver's bond kernel `kb` of launch variant v
 void launch_bond(bbpd_solver *s, KernelFn kb, int v)
 {
@@ -2250,6 +2260,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     }
     s->stencil = sp.ok;
     s->layout = sp.ok ? "stencil" : "records (" + sp.why + ")";
+    if (s->stencil && env_launch_variant() >= 0) s->occ = env_launch_variant();
 
     // ---- slots.  Stencil layout: the x-fastest lattice order.  Records: tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in Morton order,
     // lines of 64 inside a tile, so a wavefront is 64 consecutive nodes of a lattice line and the
@@ -3492,6 +3503,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     s->integrator = sh->integrator;
     s->batch = true;
     s->stencil = true;
+    if (env_launch_variant() >= 0) s->occ = env_launch_variant();
     s->lattice = true;
     s->layout = "stencil";
     s->maxRegions = spec->maxRegions;

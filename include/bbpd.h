@@ -267,7 +267,8 @@ const char *bbpd_solver_layout(bbpd_solver *s);
  * The first call runs each variant once untimed before timing it (loading its code); later calls
  * time at once.  ms (may be NULL) receives BBPD_AUTOTUNE_VARIANTS averages, choice the index kept.
  * Needs the ADR integrator (whose bond kernel leaves the state unchanged when rerun); a no-op on the
- * record layout.  No counterpart in the reference. */
+ * record layout.  BBPD_LAUNCH_VARIANT=v in the environment fixes the variant at set-up instead of 0.
+ * No counterpart in the reference. */
 #define BBPD_AUTOTUNE_VARIANTS 6
 int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *choice);
 /* average duration (ms) of the bond kernel over the steps since the last call, from HIP events

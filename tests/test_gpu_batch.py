@@ -95,6 +95,36 @@ def test_batch_adr_equals_single_runs(gpu):
         assert b.observables(ra)["cn"] != b.observables(rb)["cn"]
 
 
+def test_batch_launch_variants_agree(gpu):
+    """Every launch variant of the stencil bond kernel (block orders: XCD remap, launch order, XCD
+    stripes over the batch's block list) gives the same trajectories bit for bit, in both forms."""
+    A = adr(25, 6, 10, supports_x=(3 * DX, 23 * DX), penetrator_x=13 * DX)
+    B = adr(33, 6, 12, supports_x=(3 * DX, 31 * DX), penetrator_x=17 * DX)
+    nv = _lib.AUTOTUNE_VARIANTS
+    for prec in (_lib.FP64, _lib.MIXED):
+        ref = None
+        for v in range(nv):
+            old = os.environ.get("BBPD_LAUNCH_VARIANT")
+            os.environ["BBPD_LAUNCH_VARIANT"] = str(v)
+            try:
+                b = Batch(A, prec, max_regions=2, max_slots=20000)
+            finally:
+                if old is None:
+                    os.environ.pop("BBPD_LAUNCH_VARIANT")
+                else:
+                    os.environ["BBPD_LAUNCH_VARIANT"] = old
+            with b:
+                ra, rb = b.admit(A), b.admit(B)
+                b.step(300)
+                out = [b.nodal(ra), b.nodal(rb), b.bonds(ra), b.bonds(rb)]
+            if ref is None:
+                ref = out
+                continue
+            for x, y in zip(ref, out):
+                for p, q in zip(x, y):
+                    assert same(p, q), (prec, v)
+
+
 def test_batch_retire_readmit_and_graph(gpu):
     """Retiring frees the slots for the next problem; survivors are undisturbed; a captured graph
     is rebuilt when the membership changes and equals eager stepping."""

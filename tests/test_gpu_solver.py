@@ -531,6 +531,24 @@ def test_autotune_leaves_the_trajectory_unchanged(gpu):
             e.autotune(3)
 
 
+def test_launch_variants_agree(gpu):
+    """Every launch variant of the stencil bond kernel (minimum waves per SIMD; block orders: XCD
+    remap, launch order, XCD stripes) gives the same ADR trajectory bit for bit, in both forms."""
+    case = adr_case()
+    for prec in (_lib.FP64, _lib.MIXED):
+        ref = None
+        for v in range(_lib.AUTOTUNE_VARIANTS):
+            with _solver_with_env(case, prec, BBPD_LAUNCH_VARIANT=str(v)) as a:
+                assert a.layout().startswith("stencil")
+                a.step(300)
+                out = list(a.nodal()) + list(a.bonds())
+            if ref is None:
+                ref = out
+                continue
+            for x, y in zip(ref, out):
+                assert same(x, y), (prec, v)
+
+
 @pytest.mark.parametrize("law", [_lib.LAW_EXPONENTIAL, _lib.LAW_TRILINEAR])
 def test_damage_field_and_energy_match_oracle(gpu, oracle, law):
     """Device damage field and damage energy against the oracle, which keeps the reference's per-bond
