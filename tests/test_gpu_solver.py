@@ -371,12 +371,12 @@ def _solver_with_env(case, precision, **env):
                 os.environ[k] = v
 
 
-@pytest.mark.parametrize("dx", [5e-3, 1.25e-3])
+@pytest.mark.parametrize("dx", [5e-3, 1.25e-3, 7e-3, 6.25e-4])
 def test_fp64_nominal_lengths_equal_per_entry(gpu, dx):
     """FP64 stencil kernel: the undeformed lengths, their reciprocals and the volume correction
     factors taken from per-offset nominal values (one correction step each, checked at set-up on
     every value they take on the lattice) give the ADR trajectory of computing them per entry with
-    square root and division, bit for bit, on an exponential-law beam at two lattice spacings."""
+    square root and division, bit for bit, on an exponential-law beam at four lattice spacings."""
     case = beam.build_beam("t", 24, 6, 10, dx, law=_lib.LAW_EXPONENTIAL, s0=1e-4, integrator=_lib.ADR,
                            iters_per_increment=100, increment=-4e-3 * dx,
                            notch=(12.5, 3.5))
