@@ -19,7 +19,9 @@ Reported (one JSON line on stdout):
                    model); `frac_layout_bytes` is the same with the stencil layout's own bytes per
                    launch; `traffic` is the kernel's measured HBM bytes per launch (rocprofv3
                    FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's reading for a device
-                   copy of known size)
+                   copy of known size) and `traffic_gbs` that per launch time; `copy_gbs` is this
+                   GPU's device-to-device copy rate (bytes read plus written), the on-box peak beside
+                   the published one
   cpu_baseline     the oracle's OpenMP restatement of the reference CPU passes timed on the host on
                    a bounded number of steps of the same workload
 """
@@ -116,6 +118,26 @@ def run_form(case, precision, steps, warmup, chunk=None, keep=(), retune=False):
                 kernel_ms=avg_ms, launches=launches, bytes_model=model, bytes_layout=info["bond_kernel_bytes"],
                 achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS, info=info, hist=hist, ckpts=ckpts,
                 layout=sol.layout(), autotune=tune, retune_picks=picks)
+
+
+def copy_bandwidth(gib: float = 2.0, reps: int = 10) -> float:
+    """This GPU's device-to-device copy rate (GB/s, bytes read plus bytes written) over a buffer of
+    `gib` GiB: the on-box reference beside the published 8 TB/s peak."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
 
 
 def increment_curve(R: np.ndarray, ipi: int) -> np.ndarray:
@@ -461,6 +483,9 @@ def main():
     if traffic:
         log(f"traffic: {traffic['bytes'] / 1e9:.3f} GB per launch (fetch x{traffic['fetch_scale']:.2f}, "
             f"write x{traffic['write_scale']:.2f})")
+    copy_gbs = copy_bandwidth() if args.form == "both" else None
+    if copy_gbs:
+        log(f"device copy: {copy_gbs:.0f} GB/s")
     study = args.form == "both" and not args.no_precision and total >= 2 * ipi
     # FP64 states kept for the precision study: after increment 1 and every fifth increment
     keep = sorted({ipi * j for j in range(1, total // ipi + 1) if j == 1 or j % 5 == 0}) if study else ()
@@ -533,6 +558,8 @@ def main():
                      "traffic_fetch": traffic["fetch_bytes"] if traffic else None,
                      "traffic_write": traffic["write_bytes"] if traffic else None,
                      "frac_layout_bytes": head["bytes_layout"] / (head["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "copy_gbs": copy_gbs,
+                     "traffic_gbs": traffic["bytes"] / (head["kernel_ms"] * 1e-3) / 1e9 if traffic else None,
                      "kernel": f"bond kernel, FP32-mixed, {head['layout']} layout", "kernel_ms": head["kernel_ms"],
                      "autotune": {"variant": head["autotune"][0], "ms": head["autotune"][1],
                                   "retune_picks": {str(v): head["retune_picks"].count(v)
