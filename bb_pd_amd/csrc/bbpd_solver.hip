@@ -90,6 +90,7 @@ struct Region {
                                    // and the volume correction factors from the per-offset nominal
                                    // values (checked bit for bit at set-up)
     double s0, s1, sc;             // scalar law parameters
+    double s0lo, sclo;             // stencil layout: the smallest streamed s0 / sc of the region
     int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
     int schedule, nTimeSteps, ipi; // load schedule
     double applied, increment;
@@ -752,6 +753,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const bool fast64 = !MIXED && RG.fast64;
     const bool real = n - base < RG.nN;
     const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = law_threshold<LAW, T>(rs0, rsc, rs1);
+    const T rs0lo = (T)RG.s0lo, rsclo = (T)RG.sclo;
     const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
     const int cur = (int)(*P.tick & 1);
     const NodeRec *__restrict__ pos = P.pos[cur];
@@ -834,8 +836,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     auto load = [&](StEnt<T> &d, int o) {
         const uint32_t so = (uint32_t)o * (SLICE * TW);
         d.h = bload_nt<T>(rh, lb, so);
-        if (PV == PV_GENERAL && P.s0mode == PM_STREAM) d.p0 = bload_nt<T>(r0, lb, so);
-        if (PV == PV_GENERAL && P.scmode == PM_STREAM) d.p1 = bload_nt<T>(r1, lb, so);
         d.st = sload(sst, o);
         d.mk = sload(mks, o);
     };
@@ -876,8 +876,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             c = tk[0];
         }
         if (PV == PV_GENERAL) {
-            s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? d.p0 : rs0);
-            sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? d.p1 : rsc);
+            // a streamed s0 / sc enters the threshold as the region's smallest value: a stretch at or
+            // below it cannot change the law's state, and the entry's own value is read only when the
+            // law runs (most entries, most steps, read no parameter stream)
+            s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? rs0lo : rs0);
+            sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? rsclo : rsc);
             thr = law_threshold<LAW, T>(s0, sc, rs1);
         }
         // the law, and the store of a changed history (sign = flag)
@@ -888,6 +891,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
                 s0 = bload<T>(rtab, (uint32_t)(ts0 + ki) * TW);
                 sc = bload<T>(rtab, (uint32_t)(tsc + ki) * TW);
             }
+            if (PV == PV_GENERAL && P.s0mode == PM_STREAM) s0 = bload<T>(r0, lb + so);
+            if (PV == PV_GENERAL && P.scmode == PM_STREAM) sc = bload<T>(r1, lb + so);
             uint32_t w = flag ? mark : 0u;
             T H = MIXED ? (T)fabsf((float)d.h) : (T)fabs((double)d.h);
             apply_law<LAW, T>(s, s0, sc, rs1, lk, lalpha, ldenom, lsteel, lff, H, w);
@@ -2680,6 +2685,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.s0 = pr->s0 ? 0.0 : pr->s0_scalar;
     R0.sc = pr->sc ? 0.0 : pr->sc_scalar;
     R0.s1 = pr->s1;
+    R0.s0lo = pr->s0 && nB > 0 ? *std::min_element(pr->s0, pr->s0 + nB) : 0.0;
+    R0.sclo = pr->sc && nB > 0 ? *std::min_element(pr->sc, pr->sc + nB) : 0.0;
     R0.body0 = 0;
     R0.nBodies = pr->nBodies;
     R0.schedule = pr->schedule;
@@ -3850,6 +3857,8 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.s0 = pr->s0 ? 0.0 : pr->s0_scalar;
     RG.sc = pr->sc ? 0.0 : pr->sc_scalar;
     RG.s1 = pr->s1;
+    RG.s0lo = pr->s0 && nB > 0 ? *std::min_element(pr->s0, pr->s0 + nB) : 0.0;
+    RG.sclo = pr->sc && nB > 0 ? *std::min_element(pr->sc, pr->sc + nB) : 0.0;
     RG.body0 = rid * BBPD_MAX_BODIES;
     RG.nBodies = pr->nBodies;
     RG.schedule = pr->schedule;
