@@ -195,6 +195,8 @@ struct Params {
     const unsigned long long *smaskT; // per (slice, offset) in entry order: bit lane set when the slot
                                       // (slice, lane) has the bond; ST_PAD entries past the last
     void *hs;             // per entry (slice, offset, lane): bondSofteningFactor, sign bit = flag
+    unsigned long long *snz; // per slice, two words: bit o set when an entry of offset o may hold a
+                             // history word other than +0 (the others all hold +0)
     const float *tabf;    // mixed-form key tables (c * V, then parameter tables)
 };
 
@@ -831,11 +833,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     double f0 = 0, f1 = 0, f2 = 0;
     const StOff *sst = P.sst + RG.sst0;
     const unsigned long long *mks = P.smaskT + eslice / SLICE;
+    // the slice's history words (noff <= 128): an offset whose bit is clear holds +0 in every entry,
+    // which is taken without a load.  This wavefront alone writes them, after its one read.
+    unsigned long long *hzp = P.snz + 2 * (n0 / SLICE);
+    const unsigned long long hz0 = sload(hzp, 0), hz1 = sload(hzp, 1);
+    auto hbit = [&](int o) { return o < 128 && (((o < 64 ? hz0 : hz1) >> (o & 63)) & 1ull) != 0; };
     // offset o's loads; o runs to noff: the entry arrays end at least two slices' worth past the
     // last entry and the per-offset table ST_PAD records past the last offset
     auto load = [&](StEnt<T> &d, int o) {
         const uint32_t so = (uint32_t)o * (SLICE * TW);
-        d.h = bload_nt<T>(rh, lb, so);
+        if (hbit(o)) d.h = bload_nt<T>(rh, lb, so);
+        else d.h = (T)0;
         d.st = sload(sst, o);
         d.mk = sload(mks, o);
     };
@@ -899,8 +907,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             flag = (w & mark) != 0u;
             hv = flag ? -H : H;
             if (__builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, hv) !=
-                __builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, d.h))
+                __builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, d.h)) {
                 bstore<T>(hv, rh, lb, so);
+                // the first history word other than +0 at this offset of the slice
+                if (!hbit(o)) atomicOr(hzp + (o >> 6), 1ull << (o & 63));
+            }
         };
         if constexpr (!MIXED) {
             const double Xm0 = P.hx * (double)(kx + d.st.dx);
@@ -1945,6 +1956,24 @@ std::vector<unsigned long long> lane_masks(const std::vector<uint4> &m, long lon
     return r;
 }
 
+// History words of the stencil layout: per slice of 64 slots, two words whose bit o is set when an
+// entry of offset o holds a history word other than +0 (a set flag included); from the history `h`
+// of `nsl` slices in entry order (slice, offset, lane).  The bond kernel takes +0 for an offset whose
+// bit is clear without loading it, and sets the bit when it stores a history word there.
+template <typename V>
+std::vector<unsigned long long> history_words(const V *h, long long nsl, int noff)
+{
+    std::vector<unsigned long long> w((size_t)nsl * 2, 0ull);
+    for (long long q = 0; q < nsl; q++)
+        for (int o = 0; o < noff; o++) {
+            const V *e = h + (q * noff + o) * SLICE;
+            bool nz = false;
+            for (int l = 0; l < SLICE; l++) nz |= e[l] != 0 || std::signbit(e[l]);
+            if (nz) w[(size_t)q * 2 + (o >> 6)] |= 1ull << (o & 63);
+        }
+    return w;
+}
+
 // Per-offset records of the stencil layout, then ST_PAD copies of the last.
 // Mixed form: xi = h * d rounded to FP32 per component (an offset and its negation give exactly
 // negated vectors), |xi| and |xi|^2 from those, and the volume correction factor of |h * d|
@@ -2460,7 +2489,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     // stencil layout
     void *hsd = nullptr;
     uint4 *maskd = nullptr;
-    unsigned long long *maskTd = nullptr;
+    unsigned long long *maskTd = nullptr, *snzd = nullptr;
     int4 *stoffd = nullptr;
     int2 *srowd = nullptr;
     StOff *sstd = nullptr;
@@ -2469,6 +2498,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         if ((rc = dalloc((char **)&hsd, (size_t)(nE + 2 * SLICE) * (mixed ? 4 : 8), s->owned))) return fail(rc);
         if ((rc = dalloc(&maskd, nSlots, s->owned))) return fail(rc);
         if ((rc = dalloc(&maskTd, (size_t)s->nSlices * sp.noff + ST_PAD, s->owned))) return fail(rc);
+        if ((rc = dalloc(&snzd, (size_t)s->nSlices * 2, s->owned))) return fail(rc);
         if ((rc = dalloc(&stoffd, sp.noff, s->owned))) return fail(rc);
         if ((rc = dalloc(&srowd, sp.rows.size(), s->owned))) return fail(rc);
         if ((rc = dalloc(&sstd, sp.noff + ST_PAD, s->owned))) return fail(rc);
@@ -2558,11 +2588,15 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
             for (int k = 0; k < nB; k++)
                 if (pr->fail && pr->fail[k] == 0) h0[s->entry_i[k]] = h0[s->entry_j[k]] = -0.0f;
             BBPD_HIP(hipMemcpy(hsd, h0.data(), sizeof(float) * nE, hipMemcpyHostToDevice));
+            const std::vector<unsigned long long> hw = history_words(h0.data(), s->nSlices, sp.noff);
+            BBPD_HIP(hipMemcpy(snzd, hw.data(), sizeof(unsigned long long) * hw.size(), hipMemcpyHostToDevice));
         } else {
             std::vector<double> h0(nE, 0.0);
             for (int k = 0; k < nB; k++)
                 if (pr->fail && pr->fail[k] == 0) h0[s->entry_i[k]] = h0[s->entry_j[k]] = -0.0;
             BBPD_HIP(hipMemcpy(hsd, h0.data(), sizeof(double) * nE, hipMemcpyHostToDevice));
+            const std::vector<unsigned long long> hw = history_words(h0.data(), s->nSlices, sp.noff);
+            BBPD_HIP(hipMemcpy(snzd, hw.data(), sizeof(unsigned long long) * hw.size(), hipMemcpyHostToDevice));
         }
         BBPD_HIP(hipMemcpy(stoffd, sp.off.data(), sizeof(int4) * sp.noff, hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(srowd, sp.rows.data(), sizeof(int2) * sp.rows.size(), hipMemcpyHostToDevice));
@@ -2653,6 +2687,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.smask = maskd;
     P.smaskT = maskTd;
     P.hs = hsd;
+    P.snz = snzd;
     P.tabf = tabfd;
     // key tables in LDS; with s0 and sc both tabled the stencil kernel keeps (c, law threshold) per key
     {
@@ -2997,6 +3032,8 @@ int fetch_hs(bbpd_solver *s, std::vector<double> &hv)
 int put_hs(bbpd_solver *s, const std::vector<double> &hv)
 {
     const long long nE = s->nPadded;
+    const std::vector<unsigned long long> hw = history_words(hv.data(), s->nSlices, (int)(nE / s->nSlices / SLICE));
+    BBPD_HIP(hipMemcpy(s->P.snz, hw.data(), sizeof(unsigned long long) * hw.size(), hipMemcpyHostToDevice));
     if (s->precision == BBPD_PRECISION_MIXED) {
         std::vector<float> f(nE);
         for (long long e = 0; e < nE; e++) f[e] = (float)hv[e];
@@ -3539,7 +3576,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     NodeRec *posd[2];
     unsigned char *consd;
     uint4 *maskd;
-    unsigned long long *maskTd;
+    unsigned long long *maskTd, *snzd;
     void *hsd, *s0d = nullptr, *scd = nullptr;
     int4 *stoffd;
     int2 *srowd;
@@ -3558,6 +3595,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if ((rc = dalloc(&consd, nS, s->owned))) return fail(rc);
     if ((rc = dalloc(&maskd, nS, s->owned))) return fail(rc);
     if ((rc = dalloc(&maskTd, nS / SLICE * ST_MAX + ST_PAD, s->owned))) return fail(rc);
+    if ((rc = dalloc(&snzd, nS / SLICE * 2, s->owned))) return fail(rc);
     if ((rc = dalloc((char **)&hsd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if (sh->s0 && (rc = dalloc((char **)&s0d, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if (sh->sc && (rc = dalloc((char **)&scd, (size_t)(nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
@@ -3657,6 +3695,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     P.smask = maskd;
     P.smaskT = maskTd;
     P.hs = hsd;
+    P.snz = snzd;
     s->device_bytes = nS * (2 * sizeof(NodeRec) + 8 * (3 * (mixed ? 3 : 4) + 2 + (Fpd ? 3 : 0)) + 1 + 16) +
                       (size_t)nE * ew * (1 + (s0d ? 1 : 0) + (scd ? 1 : 0));
     s->ev.resize(2048);
@@ -3908,6 +3947,11 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
         return BBPD_OK;
     };
     if ((rc = put_entries(P.hs, h0))) return rc;
+    {
+        const std::vector<unsigned long long> hw = history_words(h0.data(), nSl / SLICE, noff);
+        BBPD_HIP(hipMemcpy(P.snz + 2 * (base / SLICE), hw.data(), sizeof(unsigned long long) * hw.size(),
+                           hipMemcpyHostToDevice));
+    }
     if (pr->s0 && (rc = put_entries((void *)P.s0str, hs0))) return rc;
     if (pr->sc && (rc = put_entries((void *)P.scstr, hsc))) return rc;
     BBPD_HIP(hipMemcpy((int4 *)P.soff + RG.off0, sp.off.data(), sizeof(int4) * noff, hipMemcpyHostToDevice));

@@ -269,9 +269,10 @@ def cpu_baseline(case, target_s=20.0):
                        f"or_step_omp with {threads} OpenMP threads")
 
 
-def _pmc(counter: str, args: list, kernel_match: str):
+def _pmc(counter: str, args: list, kernel_match: str, last: int = 0):
     """Average value of one counter per dispatch of the matching kernel, from a rocprofv3 child run
-    of this script (the program itself follows `--`)."""
+    of this script (the program itself follows `--`); `last` > 0 averages the last `last` such
+    dispatches only (the timed steps, not the warm-up or the autotuner's trial launches)."""
     exe = shutil.which("rocprofv3")
     if not exe:
         return None
@@ -282,25 +283,31 @@ def _pmc(counter: str, args: list, kernel_match: str):
             subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=240)
         except (subprocess.SubprocessError, OSError):
             return None
-        vals = []
+        per = {}
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             with open(f) as fh:
                 for r in csv.DictReader(fh):
                     if kernel_match in r["Kernel_Name"] and r["Counter_Name"] == counter:
-                        vals.append(float(r["Counter_Value"]))
+                        k = int(r["Dispatch_Id"])
+                        per[k] = per.get(k, 0.0) + float(r["Counter_Value"])
+        vals = [per[k] for k in sorted(per)]
+        if last > 0:
+            vals = vals[-last:]
         return sum(vals) / len(vals) if vals else None
 
 
 def measure_traffic(form: str):
-    """HBM bytes per launch of the bond kernel: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
-    scaled by bytes / counter for a device copy of 1 GiB (read 1 GiB, write 1 GiB)."""
+    """Bytes per launch of the bond kernel between L2 and memory over a short run's timed steps:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by bytes / counter for a device copy of
+    1 GiB (read 1 GiB, write 1 GiB).  The fetches include those that the memory-side cache serves."""
     gib = 1 << 30
     cal_args = ["--calibrate-copy-gib", "1"]
-    kargs = ["--form", form, "--steps", "5", "--warmup", "2", "--no-cpu-baseline", "--no-traffic"]
+    steps = 5
+    kargs = ["--form", form, "--steps", str(steps), "--warmup", "2", "--no-cpu-baseline", "--no-traffic"]
     cf = _pmc("FETCH_SIZE", cal_args, "copyBuffer")
     cw = _pmc("WRITE_SIZE", cal_args, "copyBuffer")
-    kf = _pmc("FETCH_SIZE", kargs, "k_bond")
-    kw = _pmc("WRITE_SIZE", kargs, "k_bond")
+    kf = _pmc("FETCH_SIZE", kargs, "k_bond", last=steps)
+    kw = _pmc("WRITE_SIZE", kargs, "k_bond", last=steps)
     if None in (cf, cw, kf, kw) or cf <= 0 or cw <= 0:
         return None
     sf, sw = gib / cf, gib / cw

@@ -145,6 +145,33 @@ def test_checkpoint_resume_bitwise(gpu, integrator):
         assert same(got[k], ref[k]), k
 
 
+def test_restore_rewrites_history_words(gpu):
+    """The stencil bond kernel takes the history of an offset as +0, without loading it, when the
+    slice's history words say that every entry there holds +0; the host rewrites the words with every
+    history it uploads.  A damaged state restored into a fresh solver (no damage yet) and an undamaged
+    state restored over a damaged one both continue exactly as the uninterrupted runs."""
+    case = ec_case(_lib.LAW_BILINEAR)
+    with Solver(case, _lib.FP64) as a:
+        assert a.layout().startswith("stencil")
+        c0 = a.checkpoint()
+        a.step(200)
+        ck = a.checkpoint()
+        a.step(100)
+        ref = a.checkpoint()
+        assert (ck["H"] > 0).sum() > 0 and ck["flag"].sum() > 0
+        # undamaged state over a damaged solver, then the same 300 steps as from set-up
+        a.restore(c0.pop("step"), **c0)
+        a.step(300)
+        again = a.checkpoint()
+    with Solver(case, _lib.FP64) as b:
+        b.restore(ck.pop("step"), **ck)
+        b.step(100)
+        got = b.checkpoint()
+    for k in ("u", "v", "x", "F", "H", "flag", "fail"):
+        assert same(got[k], ref[k]), k
+        assert same(again[k], ref[k]), k
+
+
 def adr_case(ipi=400):
     """Symmetric 25 x 6 x 10 beam on node-column supports under ADR, loaded in increments of 2 um per
     `ipi` iterations.  Frictionless cylinders leave rigid x translation and rotation about z free,
