@@ -361,6 +361,25 @@ def test_stencil_layout_equals_records(gpu, law):
             assert oa["n_softening"] > 0
 
 
+def test_stencil_bonds_failed_at_setup_equal_records(gpu):
+    """Bonds given as failed at set-up (brittle law, fail = 0) hold -0 in the stencil layout's history,
+    so set-up must mark their offsets in the slices' history words: the bond kernel takes +0 (intact)
+    without a load for an offset whose bit is clear.  Both layouts give the same FP64 trajectory."""
+    case = ec_case(_lib.LAW_LINEAR)
+    case.fail = np.where(np.arange(case.nBonds) % 7 == 3, 0.0, 1.0)
+    a = _solver_with_layout(case, _lib.FP64, "stencil")
+    b = _solver_with_layout(case, _lib.FP64, "records")
+    with a, b:
+        assert a.layout() == "stencil", a.layout()
+        assert a.observables()["n_failed"] == int((case.fail == 0).sum())
+        a.step(150)
+        b.step(150)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+
+
 @pytest.mark.parametrize("horizon_factor", [4.6, 5.9])
 def test_stencil_layout_wide_rows(gpu, horizon_factor):
     """A one-node-thick plate with a horizon of 4.6 or 5.9 spacings: bond offsets reach 4 or 5
