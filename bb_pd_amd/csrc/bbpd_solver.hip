@@ -834,7 +834,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const StOff *sst = P.sst + RG.sst0;
     const unsigned long long *mks = P.smaskT + eslice / SLICE;
     // the slice's history words (noff <= 128): an offset whose bit is clear holds +0 in every entry,
-    // which is taken without a load.  This wavefront alone writes them, after its one read.
+    // which is taken without a load.  This wavefront alone writes them, and only the bit of an offset
+    // it has already loaded, so a scalar read of them at any point of the wavefront (the compiler may
+    // repeat it: the scalar path treats the words as constant) gives the same bits for the offsets
+    // still ahead.
     unsigned long long *hzp = P.snz + 2 * (n0 / SLICE);
     const unsigned long long hz0 = sload(hzp, 0), hz1 = sload(hzp, 1);
     auto hbit = [&](int o) { return o < 128 && (((o < 64 ? hz0 : hz1) >> (o & 63)) & 1ull) != 0; };
