@@ -16,6 +16,7 @@ EULER_CROMER, ADR = 0, 1
 FP64, MIXED = 0, 1
 MAX_BODIES = 8
 AUTOTUNE_VARIANTS = 6  # BBPD_AUTOTUNE_VARIANTS
+HISTORY_COLUMNS = 8  # BBPD_HISTORY_COLUMNS
 
 _dp = C.POINTER(C.c_double)
 
@@ -46,6 +47,7 @@ class Problem(C.Structure):
         ("schedule", C.c_int), ("nTimeSteps", C.c_int), ("appliedDisplacement", C.c_double),
         ("increment", C.c_double), ("itersPerIncrement", C.c_int),
         ("referenceNode", C.c_int), ("cmodNode1", C.c_int), ("cmodNode2", C.c_int),
+        ("eqTol", C.c_double), ("itersPerIncrementMin", C.c_int),
     ]
 
 
@@ -72,6 +74,16 @@ class RegionStatus(C.Structure):
 
 
 ERR_FULL = -5
+ERR_NOMEM = -3
+
+HISTORY_KEYS = ("reaction", "reaction_static", "deflection", "cmod", "disp", "ratio", "cn", "inc_end")
+
+
+def history_dict(rows):
+    """Per-step history rows (BBPD_HISTORY_COLUMNS doubles each) as named columns."""
+    d = {k: rows[:, i].copy() for i, k in enumerate(HISTORY_KEYS)}
+    d["inc_end"] = d["inc_end"] != 0
+    return d
 
 # name -> (restype, argtypes); every symbol include/bbpd.h declares
 _V, _I, _D, _L, _P = C.c_void_p, C.c_int, C.c_double, C.c_int64, C.c_void_p
@@ -104,6 +116,8 @@ SIGNATURES = {
     "bbpd_solver_checkpoint": (_I, [_V, C.POINTER(_L), _P, _P, _P, _P, _P, _P, _P, _P]),
     "bbpd_solver_restore": (_I, [_V, _L, _P, _P, _P, _P, _P, _P, _P, _P]),
     "bbpd_solver_history": (_I, [_V, _P, _I, C.POINTER(_I)]),
+    "bbpd_solver_get_load_state": (_I, [_V, C.POINTER(_L), C.POINTER(_L)]),
+    "bbpd_solver_set_load_state": (_I, [_V, _L, _L]),
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_layout": (C.c_char_p, [_V]),
     "bbpd_solver_autotune": (_I, [_V, _I, _P, C.POINTER(_I)]),

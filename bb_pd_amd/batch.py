@@ -20,6 +20,10 @@ class BatchFull(_lib.BBPDError):
     """No region or slot range is free for the problem now."""
 
 
+class BatchTooSmall(_lib.BBPDError):
+    """The problem needs more node slots than the whole batch holds: it can never be admitted."""
+
+
 class Batch:
     def __init__(self, shape: BeamCase, precision: int = _lib.MIXED, max_regions: int = 64,
                  max_slots: int = 4 << 20, history_rows: int = 65536, device: int = 0):
@@ -63,6 +67,8 @@ class Batch:
                                       C.byref(r))
         if rc == _lib.ERR_FULL:
             raise BatchFull(self._L.bbpd_last_error().decode())
+        if rc == _lib.ERR_NOMEM:
+            raise BatchTooSmall(self._L.bbpd_last_error().decode())
         _lib.check(rc, "bbpd_batch_admit")
         self.cases[r.value] = case
         return r.value
@@ -101,13 +107,12 @@ class Batch:
         return {f: getattr(o, f) for f, _ in o._fields_}
 
     def history(self, region: int, cap: int = 1 << 20) -> dict:
-        rows = np.zeros((cap, 4))
+        rows = np.zeros((cap, _lib.HISTORY_COLUMNS))
         n = C.c_int()
         _lib.check(self._L.bbpd_batch_history(self._h, region, rows.ctypes.data_as(C.c_void_p), cap, C.byref(n)),
                    "bbpd_batch_history")
         r = rows[:n.value]
-        return dict(reaction=r[:, 0].copy(), reaction_static=r[:, 1].copy(), deflection=r[:, 2].copy(),
-                    cmod=r[:, 3].copy())
+        return _lib.history_dict(r)
 
     def nodal(self, region: int):
         nN = self.cases[region].nNodes

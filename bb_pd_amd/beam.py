@@ -89,6 +89,8 @@ class BeamCase:
     appliedDisplacement: float = -1e-3
     increment: float = -5e-6
     itersPerIncrement: int = 1000
+    itersPerIncrementMin: int = 0
+    eqTol: float = 0.0
     referenceNode: int = 0
     cmod: tuple = (0, 0)
 
@@ -99,14 +101,6 @@ class BeamCase:
     @property
     def nBonds(self) -> int:
         return self.BONDLIST.shape[0]
-
-    def displacement(self, i: int) -> float:
-        """Total load-point displacement applied at step i (1-based)."""
-        if self.schedule == 0:
-            xi = (i - 1) / (self.nTimeSteps - 1)
-            return 0 + (self.appliedDisplacement - 0) * (xi * xi * xi) * (10 - 15 * xi + 6 * (xi * xi))
-        inc = (i + self.itersPerIncrement - 1) // self.itersPerIncrement
-        return self.increment * float(inc)
 
 
 def lattice(nx, ny, nz, DX, horizon_factor=3.015, notch=None, E=37e9):
@@ -152,7 +146,8 @@ def build_beam(name: str, nx: int, ny: int, nz: int, DX: float, *, law: int, hor
                alpha: float = 0.25, integrator: int = _lib.ADR, supports_x=None, penetrator_x=None,
                schedule: int = 1, increment: float = -5e-6, iters_per_increment: int = 1000,
                nTimeSteps: int = 20000, appliedDisplacement: float = -1e-3,
-               pin_load_line: bool | None = None) -> BeamCase:
+               pin_load_line: bool | None = None, eq_tol: float = 0.0,
+               iters_per_increment_min: int = 0) -> BeamCase:
     """Three-point-bend beam in the reference's arrays.  `pin_load_line` (default: on under ADR)
     holds u_x of the top node line under the penetrator and u_y of its middle node through
     CONSTRAINTFLAG: frictionless cylinders leave rigid x translation and rotation about z without
@@ -220,7 +215,8 @@ def build_beam(name: str, nx: int, ny: int, nz: int, DX: float, *, law: int, hor
         CONSTRAINTFLAG=cons, c0=c0, E=E, law=law, s0=s0v,
         sc=scv if scv is not None else 0.0, s1=s1, k=k, alpha=alpha, integrator=integrator, DT=DT, mass=mass,
         bodies=bodies, schedule=schedule, nTimeSteps=nTimeSteps, appliedDisplacement=appliedDisplacement,
-        increment=increment, itersPerIncrement=iters_per_increment, referenceNode=ref,
+        increment=increment, itersPerIncrement=iters_per_increment, itersPerIncrementMin=iters_per_increment_min,
+        eqTol=eq_tol, referenceNode=ref,
         cmod=(nid(ixl, iy, 0), nid(ixr, iy, 0)))
 
 
@@ -319,4 +315,5 @@ def to_problem(case: BeamCase, keep: list) -> _lib.Problem:
                                                                case.itersPerIncrement)
     P.referenceNode = case.referenceNode
     P.cmodNode1, P.cmodNode2 = case.cmod
+    P.eqTol, P.itersPerIncrementMin = case.eqTol, case.itersPerIncrementMin
     return P

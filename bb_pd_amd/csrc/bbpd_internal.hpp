@@ -64,6 +64,30 @@ __device__ inline void block_sum_fixed2(double &a, double &b, double *sh)
     __syncthreads();
 }
 
+// Three block sums in one pass of the same fixed tree (each equals block_sum_fixed of its value);
+// sh holds 3 * BLOCK doubles.
+template <int BLOCK>
+__device__ inline void block_sum_fixed3(double &a, double &b, double &c, double *sh)
+{
+    sh[threadIdx.x] = a;
+    sh[BLOCK + threadIdx.x] = b;
+    sh[2 * BLOCK + threadIdx.x] = c;
+    __syncthreads();
+#pragma unroll
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+            sh[BLOCK + threadIdx.x] = sh[BLOCK + threadIdx.x] + sh[BLOCK + threadIdx.x + w];
+            sh[2 * BLOCK + threadIdx.x] = sh[2 * BLOCK + threadIdx.x] + sh[2 * BLOCK + threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    a = sh[0];
+    b = sh[BLOCK];
+    c = sh[2 * BLOCK];
+    __syncthreads();
+}
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline unsigned grid_for(long n, int block)

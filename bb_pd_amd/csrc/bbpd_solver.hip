@@ -51,7 +51,8 @@ constexpr uint32_t TYPE_SHIFT = 28;
 constexpr uint32_t FAILED_BIT = 1u << 30;
 constexpr uint32_t FLAG_BIT = 1u << 31;
 constexpr int HIST_CAP = 1 << 20;
-constexpr int N_HIST = 4;      // reaction, static reaction, deflection, CMOD
+constexpr int N_HIST = 8;      // reaction, static reaction, deflection, CMOD, load-point displacement,
+                               // equilibrium ratio, damping coefficient, 1 at a load-increment end
 constexpr int TAB_MAX = 2048;  // entries of one LDS parameter table
 constexpr int LAT_BITS = 18;   // bits per packed lattice coordinate
 constexpr uint64_t LAT_MASK = (1ull << LAT_BITS) - 1;
@@ -92,8 +93,10 @@ struct Region {
     double s0, s1, sc;             // scalar law parameters
     double s0lo, sclo;             // stencil layout: the smallest streamed s0 / sc of the region
     int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
-    int schedule, nTimeSteps, ipi; // load schedule
-    double applied, increment;
+    int schedule, nTimeSteps, ipi; // load schedule (ipi: iterations per increment, at most, for the
+    double applied, increment;     // equilibrium staircase)
+    double eq_tol;                 // equilibrium staircase: advance below this equilibrium ratio ...
+    int ipi_min;                   // ... after at least this many iterations at the increment
     int ref_slot, cmod1, cmod2;    // observation slots, -1 = none
     long long hist0;               // history ring rows hist0 .. hist0 + hist_cap - 1
     int hist_cap;
@@ -105,6 +108,9 @@ struct RegionState {
     long long step;     // steps completed
     unsigned int done;  // step-end blocks of this region finished this step
     double cn, cn1, cn2;
+    double res2;        // sum of F_d^2 over the free DOFs (the equilibrium ratio's numerator squared)
+    long long inc;      // equilibrium staircase: the current load increment (from 1) ...
+    long long inc_iters;// ... and the iterations done at it
     double reaction[BBPD_MAX_BODIES];
     double reaction_static;
     double peak;        // largest static reaction at an increment end
@@ -172,7 +178,8 @@ struct Params {
     double rtwo_r;                      // RN(1 / two_r)
     double k, alpha, denom_exp, sc_steel, ff;
     double DT, D;
-    double *partials;     // per global block: the ADR Rayleigh partial sums
+    double *partials;     // per global block, 4 doubles: the ADR Rayleigh partial sums cn1, cn2, the
+                          // squared out-of-balance force over the free DOFs, 0
     double *hist;         // history rows (N_HIST doubles each), regions own ranges of rows
     const int *active;    // active regions (one k_adr_cn block each)
     // regions
@@ -307,8 +314,10 @@ __device__ __forceinline__ double *adr_force(const Params &P, int set, int d)
 }
 
 // After a node's row: the Euler-Cromer update of the node (timeintegrationeulercromerMex.c:38-46),
-// writing its next position record, or, for ADR, F and the block's Rayleigh-quotient partial sums
-// (calculatedampingcoefficient.m:61-85).  Every thread of the block calls it (block reduction).
+// writing its next position record, or, for ADR, F and the block's partial sums: the Rayleigh
+// quotient's two (calculatedampingcoefficient.m:61-85) and the squared out-of-balance force over the
+// free DOFs (the equilibrium ratio's numerator, after calculateequilibriumstate.m:6-8).  Every
+// thread of the block calls it (block reduction; sh holds 3 * BLOCK doubles).
 template <bool MIXED>
 __device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool real, int cur, double f0, double f1,
                                             double f2, double qn0, double qn1, double qn2, double Xn0, double Xn1,
@@ -347,7 +356,7 @@ __device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool
         po->y = out[1];
         po->z = out[2];
     } else {
-        double c1 = 0, c2 = 0;
+        double c1 = 0, c2 = 0, c3 = 0;
         if (real) {
             adr_force(P, cur, 0)[n] = f0;
             adr_force(P, cur, 1)[n] = f1;
@@ -368,17 +377,19 @@ __device__ __forceinline__ void finish_node(const Params &P, int n, int gb, bool
             } else {
                 uq[0] = P.u0[n], uq[1] = P.u1[n], uq[2] = P.u2[n];
             }
+            const unsigned char cf = P.cons[n];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 if (vp[d] != 0) c1 = c1 - (uq[d] * uq[d]) * ((ff[d] / mi - fp[d] / mi) / (P.DT * vp[d]));
                 c2 = c2 + uq[d] * uq[d];
+                if (!(cf & (1u << d))) c3 = c3 + ff[d] * ff[d];
             }
         }
-        const double s1 = bbpd::block_sum_fixed<BLOCK>(c1, sh);
-        const double s2 = bbpd::block_sum_fixed<BLOCK>(c2, sh);
+        bbpd::block_sum_fixed3<BLOCK>(c1, c2, c3, sh);
         if (threadIdx.x == 0) {
-            P.partials[2 * gb] = s1;
-            P.partials[2 * gb + 1] = s2;
+            double2 *pp = reinterpret_cast<double2 *>(P.partials) + 2 * gb;
+            pp[0] = make_double2(c1, c2);
+            pp[1] = make_double2(c3, 0.0);
         }
     }
 }
@@ -430,7 +441,7 @@ template <bool MIXED, int LAW, bool LAT>
 __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[3 * BLOCK];
     extern __shared__ double tabs[]; // FP64 parameter tables
     const int gb = launch_block(P);
     if (gb < 0) return;
@@ -1056,19 +1067,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
 // (calculatedampingcoefficient.m:90-112); one block per region, a fixed summation order
 __global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P)
 {
-    __shared__ double sh[2 * BLOCK];
+    __shared__ double sh[3 * BLOCK];
     const int ri = P.active[blockIdx.x];
     const Region &RG = P.reg[ri];
-    double c1 = 0, c2 = 0;
+    double c1 = 0, c2 = 0, c3 = 0;
     // the loads of eight steps are issued together; each thread still adds its partials in order
-    const double2 *part = reinterpret_cast<const double2 *>(P.partials) + RG.blk0;
+    const double2 *part = reinterpret_cast<const double2 *>(P.partials) + 2 * (size_t)RG.blk0;
 #pragma unroll 8
     for (int q = threadIdx.x; q < RG.nblk; q += BLOCK) {
-        const double2 p = part[q];
+        const double2 p = part[2 * q], r = part[2 * q + 1];
         c1 = c1 + p.x;
         c2 = c2 + p.y;
+        c3 = c3 + r.x;
     }
-    bbpd::block_sum_fixed2<BLOCK>(c1, c2, sh);
+    bbpd::block_sum_fixed3<BLOCK>(c1, c2, c3, sh);
     const double s1 = c1, s2 = c2;
     if (threadIdx.x == 0) {
         double cn = 0;
@@ -1078,6 +1090,7 @@ __global__ __launch_bounds__(BLOCK) void k_adr_cn(Params P)
         st.cn = cn;
         st.cn1 = s1;
         st.cn2 = s2;
+        st.res2 = c3;
     }
 }
 
@@ -1140,13 +1153,16 @@ __global__ __launch_bounds__(BLOCK) void k_adr_update(Params P)
 // (smoothstepdata.m:40-42 or a staircase of increments), the step's observables and step counters.
 // One block per (region, body); a region without bodies still gets one block for its observables.
 // ------------------------------------------------------------------------------------------------
-__device__ inline double schedule_disp(const Region &R, long long i)
+// load-point displacement of step i: smoothstepdata(i, 1, N, 0, a) (the total displacement at step i);
+// a staircase of `increment` every ipi iterations; or the equilibrium staircase, `increment` times
+// the region's current increment, which the step end advances once the state is in equilibrium
+__device__ inline double schedule_disp(const Region &R, const RegionState &S, long long i)
 {
     if (R.schedule == 0) {
-        // smoothstepdata(i, 1, N, 0, a): total displacement at step i
         double xi = ((double)i - 1) / ((double)R.nTimeSteps - 1);
         return 0 + (R.applied - 0) * (xi * xi * xi) * (10 - 15 * xi + 6 * (xi * xi));
     }
+    if (R.schedule == 2) return R.increment * (double)S.inc;
     long long inc = (i + R.ipi - 1) / R.ipi;
     return R.increment * (double)inc;
 }
@@ -1174,7 +1190,7 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
     double total = 0, rs = 0;
     if (bi >= 0) {
         const Body &B = P.bodies[RG.body0 + bi];
-        const double disp = schedule_disp(RG, it);
+        const double disp = schedule_disp(RG, RS, it);
         const double cx = B.cx, czm = B.cz + (B.moves ? disp : 0.0), R = B.R;
         const int nf = B.nFam;
         double part = 0; // body 0: this thread's share of the static reaction, members in order
@@ -1301,18 +1317,37 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
         const double cmod = (RG.cmod1 >= 0 && RG.cmod2 >= 0) ? comp(RG.cmod2, 0) - comp(RG.cmod1, 0) : 0.0;
         const double r0 = RG.nBodies > 0 ? vrs->reaction[0] : 0.0;
         const double rstat = vrs->reaction_static;
+        // this step's displacement: the increment is read before this block advances it
+        const double disp = RG.schedule == 2 ? RG.increment * (double)vrs->inc : schedule_disp(RG, RS, it);
+        // equilibrium ratio of this step's forces (after calculateequilibriumstate.m:6-8): the
+        // out-of-balance force over the free DOFs against the load, +infinity while the load is 0
+        const double aload = fabs(rstat);
+        const double ratio = aload > 0 ? (sqrt(vrs->res2) * P.V) / aload : INFINITY;
+        bool inc_end = false;
+        if (RG.schedule == 1) {
+            inc_end = it % RG.ipi == 0;
+        } else if (RG.schedule == 2) {
+            const long long k = vrs->inc_iters + 1;
+            inc_end = (k >= RG.ipi_min && ratio < RG.eq_tol) || k >= RG.ipi;
+            vrs->inc_iters = inc_end ? 0 : k;
+            if (inc_end) vrs->inc = vrs->inc + 1;
+        }
         double *h = P.hist + (size_t)(RG.hist0 + (it - 1) % RG.hist_cap) * N_HIST;
         h[0] = r0;
         h[1] = rstat;
         h[2] = defl;
         h[3] = cmod;
+        h[4] = disp;
+        h[5] = ratio;
+        h[6] = P.adr ? vrs->cn : 0.0;
+        h[7] = inc_end ? 1.0 : 0.0;
         // stop criterion, judged on converged states: the ends of load increments
-        if (RG.schedule == 1 && it % RG.ipi == 0 && !vrs->finished) {
-            const double load = fabs(rstat);
+        if (inc_end && !vrs->finished) {
+            const double load = aload;
             const double peak = vrs->peak;
             bool stop = false;
             if (RG.stop_frac > 0 && peak > 0 && load < RG.stop_frac * peak) stop = true;
-            if (RG.stop_disp > 0 && fabs(schedule_disp(RG, it)) >= RG.stop_disp) stop = true;
+            if (RG.stop_disp > 0 && fabs(disp) >= RG.stop_disp) stop = true;
             if (load > peak) vrs->peak = load;
             if (stop) vrs->finished = 1;
         }
@@ -2249,7 +2284,11 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     BBPD_REQUIRE(pr->law != BBPD_LAW_LINEAR || !pr->sc, "linear law takes a scalar critical stretch");
     BBPD_REQUIRE(pr->law != BBPD_LAW_LINEAR || pr->failureFunctionality == 0 || pr->failureFunctionality == 1,
                  "failureFunctionality must be 0 or 1");
+    BBPD_REQUIRE(pr->schedule >= 0 && pr->schedule <= 2, "unknown load schedule");
     BBPD_REQUIRE(pr->schedule == 0 ? pr->nTimeSteps > 1 : pr->itersPerIncrement > 0, "bad load schedule");
+    BBPD_REQUIRE(pr->schedule != 2 || (pr->integrator == BBPD_INTEGRATOR_ADR && pr->eqTol >= 0 &&
+                                       pr->itersPerIncrementMin >= 0),
+                 "the equilibrium staircase needs ADR, eqTol >= 0 and itersPerIncrementMin >= 0");
     BBPD_REQUIRE(pr->DT > 0, "DT must be positive");
 
     int ndev = 0;
@@ -2516,7 +2555,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     if ((rc = dalloc(&tabd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
     if (sp.ok && mixed && (rc = dalloc(&tabfd, ntab > 0 ? ntab : 1, s->owned))) return fail(rc);
     double *partd;
-    if ((rc = dalloc(&partd, 2 * (size_t)(nSlots / BLOCK) + 2, s->owned))) return fail(rc);
+    if ((rc = dalloc(&partd, 4 * (size_t)(nSlots / BLOCK) + 4, s->owned))) return fail(rc);
 
     if ((rc = dalloc(&s->counts, 4, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_hist, (size_t)HIST_CAP * N_HIST, s->owned))) return fail(rc);
@@ -2732,6 +2771,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.ipi = pr->itersPerIncrement > 0 ? pr->itersPerIncrement : 1;
     R0.applied = pr->appliedDisplacement;
     R0.increment = pr->increment;
+    R0.eq_tol = pr->eqTol;
+    R0.ipi_min = pr->itersPerIncrementMin;
     auto slot_or_none = [&](int id1) { return (id1 >= 1 && id1 <= nN) ? s->slot_of[id1 - 1] : -1; };
     R0.ref_slot = slot_or_none(pr->referenceNode);
     R0.cmod1 = slot_or_none(pr->cmodNode1);
@@ -2803,6 +2844,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         if ((rc = dalloc(&s->d_tick, 1, s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_tick_done, 1, s->owned))) return fail(rc);
         RegionState z{};
+        z.inc = 1;
         long long t0 = 0;
         unsigned u0 = 0;
         BBPD_HIP(hipMemcpy(s->d_reg, &R0, sizeof(Region), hipMemcpyHostToDevice));
@@ -3235,12 +3277,12 @@ extern "C" int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, 
     if (step) *step = st;
     if ((rc = bbpd_solver_get_nodal(s, u, v, F, x))) return rc;
     if (Fp) {
+        // at a step boundary the reference's ADR hand-over has made nodalForcePrevious equal to
+        // nodalForce (dynamicsolverdisplacementcontrolledADR.m:153-155): the last step's force, which
+        // is the next step's previous force; Euler-Cromer keeps none
         if (s->P.Fp0) {
-            const int cur = current_buffer(s, &rc);
-            if (rc) return rc;
-            double *fc[3], *fp[3];
-            force_sets(s, cur, fc, fp);
-            if ((rc = fetch_soa(s, fp, Fp))) return rc;
+            if (F) std::memcpy(Fp, F, sizeof(double) * 3 * (size_t)s->nN);
+            else if ((rc = bbpd_solver_get_nodal(s, nullptr, nullptr, Fp, nullptr))) return rc;
         } else {
             std::memset(Fp, 0, sizeof(double) * 3 * (size_t)s->nN);
         }
@@ -3359,6 +3401,41 @@ extern "C" int bbpd_solver_history(bbpd_solver *s, double *out_rows, int cap, in
 {
     BBPD_REQUIRE(s && n_out && cap >= 0, "bad arguments");
     return region_history(s, 0, out_rows, cap, n_out);
+}
+
+extern "C" int bbpd_solver_get_load_state(bbpd_solver *s, int64_t *inc, int64_t *iters)
+{
+    BBPD_REQUIRE(s && inc && iters, "bad arguments");
+    BBPD_HIP(hipSetDevice(s->device));
+    RegionState d;
+    BBPD_HIP(hipMemcpyAsync(&d, s->d_rst, sizeof(d), hipMemcpyDeviceToHost, s->stream));
+    BBPD_HIP(hipStreamSynchronize(s->stream));
+    const Region &RG = s->reg[0];
+    if (RG.schedule == 2) {
+        *inc = d.inc;
+        *iters = d.inc_iters;
+    } else if (RG.schedule == 1) {
+        *inc = d.step / RG.ipi + 1;
+        *iters = d.step % RG.ipi;
+    } else {
+        *inc = 1;
+        *iters = d.step;
+    }
+    return BBPD_OK;
+}
+
+extern "C" int bbpd_solver_set_load_state(bbpd_solver *s, int64_t inc, int64_t iters)
+{
+    BBPD_REQUIRE(s && inc >= 1 && iters >= 0, "bad arguments");
+    BBPD_REQUIRE(s->reg[0].schedule == 2, "the load state is set for the equilibrium staircase only");
+    BBPD_HIP(hipSetDevice(s->device));
+    BBPD_HIP(hipStreamSynchronize(s->stream));
+    RegionState d;
+    BBPD_HIP(hipMemcpy(&d, s->d_rst, sizeof(d), hipMemcpyDeviceToHost));
+    d.inc = inc;
+    d.inc_iters = iters;
+    BBPD_HIP(hipMemcpy(s->d_rst, &d, sizeof(d), hipMemcpyHostToDevice));
+    return BBPD_OK;
 }
 
 extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_bytes, int64_t *n_entries,
@@ -3607,7 +3684,7 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     if ((rc = dalloc(&sstd, (size_t)s->maxRegions * (ST_MAX + ST_PAD), s->owned))) return fail(rc);
     if ((rc = dalloc(&tabd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
     if (mixed && (rc = dalloc(&tabfd, (size_t)s->maxRegions * RB_TAB, s->owned))) return fail(rc);
-    if ((rc = dalloc(&partd, 2 * (size_t)nblk + 2, s->owned))) return fail(rc);
+    if ((rc = dalloc(&partd, 4 * (size_t)nblk + 4, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->counts, 4, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_hist, (size_t)s->maxRegions * s->histRows * N_HIST, s->owned))) return fail(rc);
     if ((rc = dalloc(&s->d_reg, s->maxRegions, s->owned))) return fail(rc);
@@ -3733,7 +3810,11 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
                  "s0 / sc must be per bond exactly when the batch's are");
     BBPD_REQUIRE(pr->integrator != BBPD_INTEGRATOR_ADR || pr->massVector, "ADR needs massVector");
     BBPD_REQUIRE(pr->nBodies >= 0 && pr->nBodies <= BBPD_MAX_BODIES, "too many contact bodies");
+    BBPD_REQUIRE(pr->schedule >= 0 && pr->schedule <= 2, "unknown load schedule");
     BBPD_REQUIRE(pr->schedule == 0 ? pr->nTimeSteps > 1 : pr->itersPerIncrement > 0, "bad load schedule");
+    BBPD_REQUIRE(pr->schedule != 2 || (pr->integrator == BBPD_INTEGRATOR_ADR && pr->eqTol >= 0 &&
+                                       pr->itersPerIncrementMin >= 0),
+                 "the equilibrium staircase needs ADR, eqTol >= 0 and itersPerIncrementMin >= 0");
     BBPD_HIP(hipSetDevice(s->device));
     int rc;
     std::vector<int> bi, bj;
@@ -3753,6 +3834,14 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     int famtot = 0;
     for (int b = 0; b < pr->nBodies; b++) famtot += pr->nFamily[b];
     BBPD_REQUIRE(famtot <= RB_FAM, "contact families too large for a batch region");
+    // a problem larger than the whole slot pool can never be admitted: a distinct error, so that a
+    // caller does not wait for slots that never free up
+    if ((long long)(nN + BLOCK - 1) / BLOCK > (long long)s->freeblk.size()) {
+        bbpd::set_error("the problem needs " + std::to_string(((long long)nN + BLOCK - 1) / BLOCK * BLOCK) +
+                        " node slots, more than the batch holds (" +
+                        std::to_string((long long)s->freeblk.size() * BLOCK) + ")");
+        return BBPD_ERR_NOMEM;
+    }
     // region and slot range
     int rid = -1;
     for (int r = 0; r < s->maxRegions && rid < 0; r++)
@@ -3908,6 +3997,8 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.ipi = pr->itersPerIncrement > 0 ? pr->itersPerIncrement : 1;
     RG.applied = pr->appliedDisplacement;
     RG.increment = pr->increment;
+    RG.eq_tol = pr->eqTol;
+    RG.ipi_min = pr->itersPerIncrementMin;
     auto slot_or_none = [&](int id1) { return (id1 >= 1 && id1 <= nN) ? base + sp.slot[id1 - 1] : -1; };
     RG.ref_slot = slot_or_none(pr->referenceNode);
     RG.cmod1 = slot_or_none(pr->cmodNode1);
@@ -3971,6 +4062,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     BBPD_HIP(hipMemcpy(s->d_reg + rid, &RG, sizeof(Region), hipMemcpyHostToDevice));
     RegionState z;
     std::memset(&z, 0, sizeof(z));
+    z.inc = 1;
     BBPD_HIP(hipMemcpy(s->d_rst + rid, &z, sizeof(z), hipMemcpyHostToDevice));
     for (int b = 0; b < need; b++) s->freeblk[blk0 + b] = 0;
     s->reg[rid] = RG;

@@ -22,19 +22,24 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib, beam
-from .batch import Batch, BatchFull
+from .batch import Batch, BatchFull, BatchTooSmall
 
 
 @dataclass
 class Request:
     rid: int
-    case: beam.BeamCase
+    params: dict | None = None   # the drawn request; its inputs are built once it has arrived
     arrival: float = 0.0         # s after the start of the run
+    case: beam.BeamCase | None = None
+    built: float = math.nan      # inputs generated (host), s after the start
     admitted: float = math.nan
     finished: float = math.nan
     region: int = -1
     steps: int = 0
     peak: float = 0.0
+    rejected: str = ""
+    name: str = ""
+    nBonds: int = 0
     history: dict = field(default_factory=dict)
 
     @property
@@ -42,21 +47,28 @@ class Request:
         return self.finished - self.arrival
 
 
-def request_case(rng: np.random.Generator, ipi: int, increment_um: float) -> beam.BeamCase:
+def request_params(rng: np.random.Generator) -> dict:
     """One request of the serving workload: depth in {50, 100, 200, 400} mm, notch ratio in
     {0, 0.2, 0.4} (a midspan notch of that fraction of the depth), E ~ U[30, 45] GPa,
-    s0 ~ U[0.8, 1.2]e-4 (scalar), bilinear law with sc = 2e-3, dx = 5 mm, ADR on the penetrator
-    staircase of `increment_um` per `ipi` iterations."""
-    depth = int(rng.choice([50, 100, 200, 400]))
-    ratio = float(rng.choice([0.0, 0.2, 0.4]))
-    E = float(rng.uniform(30e9, 45e9))
-    s0 = float(rng.uniform(0.8e-4, 1.2e-4))
+    s0 ~ U[0.8, 1.2]e-4 (scalar)."""
+    return dict(depth=int(rng.choice([50, 100, 200, 400])), ratio=float(rng.choice([0.0, 0.2, 0.4])),
+                E=float(rng.uniform(30e9, 45e9)), s0=float(rng.uniform(0.8e-4, 1.2e-4)))
+
+
+def build_request(params: dict, ipi: int, increment_um: float) -> beam.BeamCase:
+    """The request's beam: bilinear law with sc = 2e-3, dx = 5 mm, length 5 x depth, width 50 mm, ADR
+    on the penetrator staircase of `increment_um` per `ipi` iterations."""
+    depth, ratio, E, s0 = params["depth"], params["ratio"], params["E"], params["s0"]
     nz = depth // 5
     nx = 5 * nz
     notch = (nx / 2 + 0.5, ratio * nz + 0.5) if ratio > 0 else None
     return beam.build_beam(f"req_d{depth}_n{ratio}_E{E / 1e9:.1f}", nx, 10, nz, 5e-3, law=_lib.LAW_BILINEAR,
                            s0=s0, sc=2.0e-3, E=E, notch=notch, integrator=_lib.ADR, iters_per_increment=ipi,
                            increment=-increment_um * 1e-6)
+
+
+def request_case(rng: np.random.Generator, ipi: int, increment_um: float) -> beam.BeamCase:
+    return build_request(request_params(rng), ipi, increment_um)
 
 
 def slot_bytes(precision: int) -> int:
@@ -66,34 +78,62 @@ def slot_bytes(precision: int) -> int:
     return node + 128 * w
 
 
-def serve(requests: list, precision: int, hbm_budget: float, chunk: int, stop_fraction: float = 0.2,
-          stop_displacement: float = 5e-4, max_regions: int = 64, keep_history: bool = False,
-          closed_loop: bool = False, log=None) -> dict:
+def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: int = 200,
+          increment_um: float = 25.0, stop_fraction: float = 0.2, stop_displacement: float = 5e-4,
+          max_regions: int = 64, keep_history: bool = False, keep_cases: bool = False,
+          closed_loop: bool = False, lookahead: int = 2, log=None) -> dict:
     """Run the service over `requests` (their `arrival` times are honoured unless `closed_loop`,
-    in which case all are queued at t = 0).  Returns per-request times and aggregate figures."""
+    in which case all are queued at t = 0).  A request that carries only its parameters has its
+    inputs generated after it arrives: the head of the queue when it is next to be admitted, and up
+    to `lookahead` queued requests on the host while the device runs the admitted requests' steps.
+    Input generation is therefore part of its time to solution.  Once admitted, its host arrays are
+    dropped (unless `keep_cases`): its state lives on the device until it is retired.  A request that
+    needs more node slots than the whole batch holds is rejected.  Returns per-request times and
+    aggregate figures."""
     import torch
     slots = int(hbm_budget // slot_bytes(precision))
     pending = sorted(requests, key=lambda r: r.arrival)
     queue: list = []
     active: dict = {}
     done: list = []
+    rejected: list = []
     bond_steps = 0
-    with Batch(pending[0].case, precision, max_regions=max_regions, max_slots=slots,
-               history_rows=1 << 17) as b:
+    # the batch's shared scalars (law, integrator, time step, spacing, horizon) from a request of
+    # the same workload, built before the clock starts
+    shape = pending[0].case if pending[0].case is not None else build_request(pending[0].params, ipi, increment_um)
+    with Batch(shape, precision, max_regions=max_regions, max_slots=slots, history_rows=1 << 17) as b:
+        del shape
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        while pending or queue or active:
+
+        def arrivals():
             t = now()
             while pending and (closed_loop or pending[0].arrival <= t):
                 queue.append(pending.pop(0))
+
+        def build(r):
+            if r.case is None:
+                r.case = build_request(r.params, ipi, increment_um)
+                r.built = now()
+
+        while pending or queue or active:
+            arrivals()
             # admission, first come first served; a request that does not fit waits for slots
             while queue:
                 r = queue[0]
+                build(r)
                 try:
                     r.region = b.admit(r.case, stop_fraction, stop_displacement)
                 except BatchFull:
                     break
+                except BatchTooSmall as e:
+                    r.rejected = str(e)
+                    rejected.append(queue.pop(0))
+                    continue
                 r.admitted = now()
+                r.name, r.nBonds = r.case.name, r.case.nBonds
+                if not keep_cases:
+                    r.case = None
                 active[r.region] = r
                 queue.pop(0)
             if not active:
@@ -101,8 +141,12 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, stop_fr
                     time.sleep(max(0.0, pending[0].arrival - now()))
                 continue
             b.run_graph(chunk, 1)
+            # while the device steps: generate the inputs of the next queued requests
+            arrivals()
+            for r in queue[:lookahead]:
+                build(r)
             b.sync()
-            bond_steps += chunk * sum(r.case.nBonds for r in active.values())
+            bond_steps += chunk * sum(r.nBonds for r in active.values())
             st = b.status()
             for reg, r in list(active.items()):
                 if st[reg]["finished"]:
@@ -115,11 +159,15 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, stop_fr
                     del active[reg]
                     done.append(r)
                     if log:
-                        log(f"    request {r.rid} ({r.case.name}) done: {r.steps} steps, "
+                        log(f"    request {r.rid} ({r.name}) done: {r.steps} steps, "
                             f"{r.time_to_solution:.2f} s")
         wall = now()
         torch.cuda.synchronize()
     tts = np.array([r.time_to_solution for r in done])
-    return dict(requests=len(done), wall_s=wall, requests_per_s=len(done) / wall,
+    build_s = np.array([r.built - r.arrival for r in done if not math.isnan(r.built)])
+    wait_s = np.array([r.admitted - r.arrival for r in done])
+    return dict(requests=len(done), rejected=len(rejected), wall_s=wall, requests_per_s=len(done) / wall,
                 bond_updates_per_s=bond_steps / wall, p50_s=float(np.percentile(tts, 50)),
-                p99_s=float(np.percentile(tts, 99)), slots=slots, done=done)
+                p90_s=float(np.percentile(tts, 90)), p99_s=float(np.percentile(tts, 99)), max_s=float(tts.max()),
+                build_after_arrival_s_mean=float(build_s.mean()) if build_s.size else None,
+                admission_wait_s_mean=float(wait_s.mean()), slots=slots, done=done)

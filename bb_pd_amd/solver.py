@@ -81,8 +81,19 @@ class Solver:
         _lib.check(self._L.bbpd_solver_get_bonds(self._h, *p), "bbpd_solver_get_bonds")
         return tuple(arrs)
 
+    def load_state(self):
+        """(current load increment, iterations done at it)."""
+        inc, it = C.c_int64(), C.c_int64()
+        _lib.check(self._L.bbpd_solver_get_load_state(self._h, C.byref(inc), C.byref(it)),
+                   "bbpd_solver_get_load_state")
+        return inc.value, it.value
+
+    def set_load_state(self, inc: int, iters: int):
+        _lib.check(self._L.bbpd_solver_set_load_state(self._h, int(inc), int(iters)), "bbpd_solver_set_load_state")
+
     def checkpoint(self) -> dict:
-        """Device state in the reference layout (see bbpd_solver_checkpoint)."""
+        """Device state in the reference layout (see bbpd_solver_checkpoint); under the equilibrium
+        staircase also `load_state` (increment, iterations done at it)."""
         nN, nB = self.case.nNodes, self.case.nBonds
         st = {k: np.zeros((nN, 3), order="F") for k in ("u", "v", "x", "F", "Fp")}
         st.update({k: np.zeros(nB) for k in ("H", "flag", "fail")})
@@ -90,10 +101,13 @@ class Solver:
         p = [st[k].ctypes.data_as(C.c_void_p) for k in ("u", "v", "x", "F", "Fp", "H", "flag", "fail")]
         _lib.check(self._L.bbpd_solver_checkpoint(self._h, C.byref(step), *p), "bbpd_solver_checkpoint")
         st["step"] = step.value
+        if self.case.schedule == 2:
+            st["load_state"] = self.load_state()
         return st
 
-    def restore(self, step: int, **arrays):
-        """Load state (keys u, v, x, F, Fp, H, flag, fail; missing keys stay unchanged)."""
+    def restore(self, step: int, load_state=None, **arrays):
+        """Load state (keys u, v, x, F, Fp, H, flag, fail; missing keys stay unchanged); under the
+        equilibrium staircase `load_state` = (increment, iterations done at it)."""
         keep = []
 
         def ptr(k):
@@ -106,17 +120,20 @@ class Solver:
 
         p = [ptr(k) for k in ("u", "v", "x", "F", "Fp", "H", "flag", "fail")]
         _lib.check(self._L.bbpd_solver_restore(self._h, int(step), *p), "bbpd_solver_restore")
+        if load_state is not None:
+            self.set_load_state(*load_state)
 
     def history(self, cap: int = 1 << 20) -> dict:
         """Per-step record since the previous call: arrays `reaction` (contact reaction of body 0),
-        `reaction_static` (sum of F_z * cellVolume over body 0's family), `deflection`, `cmod`."""
-        rows = np.zeros((cap, 4))
+        `reaction_static` (sum of F_z * cellVolume over body 0's family), `deflection`, `cmod`,
+        `disp` (load-point displacement), `ratio` (equilibrium ratio), `cn` (ADR damping coefficient)
+        and `inc_end` (True where the step ended a load increment)."""
+        rows = np.zeros((cap, _lib.HISTORY_COLUMNS))
         n = C.c_int()
         _lib.check(self._L.bbpd_solver_history(self._h, rows.ctypes.data_as(C.c_void_p), cap, C.byref(n)),
                    "bbpd_solver_history")
         r = rows[:n.value]
-        return dict(reaction=r[:, 0].copy(), reaction_static=r[:, 1].copy(), deflection=r[:, 2].copy(),
-                    cmod=r[:, 3].copy())
+        return _lib.history_dict(r)
 
     def damage(self, energy: bool = True):
         """(damage field per node in the reference order, total damage energy or None)."""

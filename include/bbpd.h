@@ -179,8 +179,15 @@ typedef struct bbpd_problem {
     int nFamily[BBPD_MAX_BODIES];
     double centreX[BBPD_MAX_BODIES], centreZ[BBPD_MAX_BODIES], radius[BBPD_MAX_BODIES];
     int moves[BBPD_MAX_BODIES]; /* 1: centre follows the load schedule */
-    /* load schedule: 0 = smoothstepdata(i, 1, nTimeSteps, 0, appliedDisplacement);
-     * 1 = staircase: -increment * ceil(i / itersPerIncrement) signed by appliedDisplacement */
+    /* load schedule, the load-point displacement of step i:
+     * 0 = smoothstepdata(i, 1, nTimeSteps, 0, appliedDisplacement);
+     * 1 = staircase: increment * ceil(i / itersPerIncrement);
+     * 2 = equilibrium staircase (ADR): increment * inc, where inc (from 1) advances after the step at
+     *     whose end the equilibrium ratio is below eqTol, once at least itersPerIncrementMin
+     *     iterations ran at inc, or after itersPerIncrement iterations at inc.  The equilibrium ratio
+     *     (after calculateequilibriumstate.m:6-8) is sqrt(sum of F_d^2 over the DOFs that neither
+     *     CONSTRAINTFLAG nor a contact body holds) * cellVolume / |static reaction| of the step's
+     *     forces, +infinity while the reaction is 0. */
     int schedule;
     int nTimeSteps;
     double appliedDisplacement;
@@ -189,6 +196,9 @@ typedef struct bbpd_problem {
     /* observables */
     int referenceNode; /* 1-based; 0 = none */
     int cmodNode1, cmodNode2;
+    /* equilibrium staircase (schedule 2) */
+    double eqTol;
+    int itersPerIncrementMin;
 } bbpd_problem;
 
 typedef struct bbpd_solver bbpd_solver;
@@ -225,17 +235,32 @@ int bbpd_solver_get_bonds(bbpd_solver *s, double *bondSofteningFactor, double *f
  * Fp = nodalForcePrevious for ADR), bond arrays nBonds in BONDLIST order (H =
  * bondSofteningFactor, flag = flagBondSoftening, fail), `step` = steps completed
  * (timeStepTracker - 1).  restore() takes the same arrays; NULL leaves that part unchanged.
+ * Under ADR, F is the last step's force, which the next step takes as its previous force: at a
+ * step boundary the reference's hand-over (dynamicsolverdisplacementcontrolledADR.m:153-155) has
+ * made nodalForcePrevious equal nodalForce, so checkpoint() returns Fp equal to F, and restore()
+ * takes the previous force from F (Fp is accepted for symmetry and has no effect on the steps).
  * The FP32-mixed form keeps displacements, not coordinates: its x is X + u on output and ignored
  * on input. */
 int bbpd_solver_checkpoint(bbpd_solver *s, int64_t *step, double *u, double *v, double *x, double *F, double *Fp,
                            double *H, double *flag, double *fail);
 int bbpd_solver_restore(bbpd_solver *s, int64_t step, const double *u, const double *v, const double *x,
                         const double *F, const double *Fp, const double *H, const double *flag, const double *fail);
-/* Per-step record since the previous call, 4 doubles per step in this order: body-0 contact
- * reaction (calculatecontactforce.m:121), static reaction (calculatereactionforceFast.m:3-5 over the
- * body-0 family), deflection u_z(referenceNode), CMOD u_x(cmod2) - u_x(cmod1).  At most `cap` steps
- * are returned (the device keeps the last 2^20). */
+/* Per-step record since the previous call, BBPD_HISTORY_COLUMNS doubles per step in this order:
+ * body-0 contact reaction (calculatecontactforce.m:121), static reaction
+ * (calculatereactionforceFast.m:3-5 over the body-0 family, from the step's forces), deflection
+ * u_z(referenceNode), CMOD u_x(cmod2) - u_x(cmod1) (dynamicsolverdisplacementcontrolled.m:115-124),
+ * the step's load-point displacement, its equilibrium ratio (see schedule 2), the ADR damping
+ * coefficient (0 under Euler-Cromer), and 1 when the step ended a load increment (its static
+ * reaction is then the increment's converged load), else 0.  At most `cap` steps are returned (the
+ * device keeps the last 2^20). */
+#define BBPD_HISTORY_COLUMNS 8
 int bbpd_solver_history(bbpd_solver *s, double *rows, int cap, int *n_out);
+/* Load state of the equilibrium staircase: the current increment (from 1) and the iterations done at
+ * it.  get returns them (and, for the other schedules, the increment and iteration count the step
+ * counter implies); set loads them, after bbpd_solver_restore, for schedule 2.  No counterpart in the
+ * reference, whose load schedule is a function of the step (smoothstepdata.m:40-42). */
+int bbpd_solver_get_load_state(bbpd_solver *s, int64_t *increment, int64_t *itersAtIncrement);
+int bbpd_solver_set_load_state(bbpd_solver *s, int64_t increment, int64_t itersAtIncrement);
 /* Damage field (calculatedamage.m:34-45: 1 - unbroken / nFAMILY per node, broken meaning H == 1 or
  * failed; damage is nNodes, host, reference order, may be NULL) and total dissipated damage energy
  * (calculatedamageenergy.m:35-62 summed over bonds, with the solver's k and alpha; may be NULL).
@@ -311,7 +336,8 @@ int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int device, bb
 /* Admit a problem: it starts at step 0 with the next launch.  stopFraction / stopDisplacement (0 =
  * none): the problem is marked finished at the first load-increment end where its static reaction
  * is below stopFraction of its peak, or the load point has travelled stopDisplacement.  Returns
- * BBPD_ERR_FULL when no region or slot range is free now. */
+ * BBPD_ERR_FULL when no region or slot range is free now, BBPD_ERR_NOMEM when the problem needs more
+ * node slots than the whole batch holds (it can never be admitted). */
 int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *problem, double stopFraction, double stopDisplacement,
                      int *region);
 int bbpd_batch_retire(bbpd_solver *s, int region);

@@ -296,19 +296,107 @@ double or_calculatedampingcoefficient(const double *F, const double *m, const do
     return or_calculatedampingcoefficient_masked(F, m, Fp, DT, vp, u, nNodes, NULL);
 }
 
-/* Same quantity with the two sums returned, for checking a GPU reduction (a different order). */
-void or_dampingsums(const double *F, const double *m, const double *Fp, double DT, const double *vp,
-                    const double *u, int nNodes, double *cn1_out, double *cn2_out)
+/* ------------------------------------------------------------------------------------------------
+ * The device's summation orders.  Not reference routines: the reference sums sequentially
+ * (calculatedampingcoefficient.m:61-85, MATLAB's sum in calculatereactionforceFast.m:3-5).  They let
+ * the oracle take the same sums in the order the GPU's fixed-shape reductions add their terms, so
+ * that trajectories whose damping coefficient, static reaction or equilibrium ratio feeds back into
+ * the state (ADR, the equilibrium staircase) can be compared bit for bit.
+ *   or_tree_sum   n values (n a power of two) by the halving tree v[t] = v[t] + v[t + w] for
+ *                 w = n/2, n/4, .., 1 and t < w; returns v[0] (v is overwritten)
+ *   or_slot_sum   one value per node slot (nSlots a multiple of 256): each block of 256 consecutive
+ *                 slots by the halving tree; then lane t of one 256-lane block adds the block sums
+ *                 t, t + 256, t + 512, .. in that order starting from 0, and the 256 lanes are
+ *                 summed by the halving tree
+ * ---------------------------------------------------------------------------------------------- */
+double or_tree_sum(double *v, int n)
 {
-    long double cn1 = 0, cn2 = 0;
-    for (int i = 0; i < nNodes; i++)
+    for (int w = n / 2; w > 0; w >>= 1)
+        for (int t = 0; t < w; t++) v[t] = v[t] + v[t + w];
+    return v[0];
+}
+
+double or_slot_sum(const double *by_slot, long nSlots)
+{
+    long nblk = nSlots / 256;
+    double *part = (double *)malloc(sizeof(double) * (size_t)(nblk > 0 ? nblk : 1));
+    double buf[256];
+    for (long b = 0; b < nblk; b++) {
+        memcpy(buf, by_slot + 256 * b, sizeof(buf));
+        part[b] = or_tree_sum(buf, 256);
+    }
+    for (int t = 0; t < 256; t++) {
+        double c = 0;
+        for (long q = t; q < nblk; q += 256) c = c + part[q];
+        buf[t] = c;
+    }
+    free(part);
+    return or_tree_sum(buf, 256);
+}
+
+/* The ADR sums of calculatedampingcoefficient.m:61-85 per node (massVector(i,1) for every DOF, the
+ * terms of a node added d = 1..3 from 0), over the nodes with skip == 0 (NULL = all), placed at the
+ * node's slot and summed by or_slot_sum: out[0] = cn1, out[1] = cn2.  out[2] is the squared
+ * out-of-balance force sum F_d^2 over the DOFs that neither CONSTRAINTFLAG (cons, nNodes x 3, may be
+ * NULL) nor skip holds, the numerator of the equilibrium ratio (see or_equilibrium_ratio). */
+void or_dampingsums_device(const double *F, const double *m, const double *Fp, double DT, const double *vp,
+                           const double *u, int nNodes, const unsigned char *skip, const double *cons,
+                           const int *slot_of, long nSlots, double *out)
+{
+    double *v[3];
+    for (int k = 0; k < 3; k++) v[k] = (double *)calloc((size_t)nSlots, sizeof(double));
+    for (int i = 0; i < nNodes; i++) {
+        if (skip && skip[i]) continue;
+        double c1 = 0, c2 = 0, c3 = 0;
         for (int d = 0; d < 3; d++) {
             size_t q = i + (size_t)nNodes * d;
-            if (vp[q] != 0) cn1 -= (long double)((u[q] * u[q]) * ((F[q] / m[i] - Fp[q] / m[i]) / (DT * vp[q])));
-            cn2 += (long double)(u[q] * u[q]);
+            if (vp[q] != 0) c1 = c1 - (u[q] * u[q]) * ((F[q] / m[i] - Fp[q] / m[i]) / (DT * vp[q]));
+            c2 = c2 + u[q] * u[q];
+            if (!(cons && cons[q] == 1)) c3 = c3 + F[q] * F[q];
         }
-    *cn1_out = (double)cn1;
-    *cn2_out = (double)cn2;
+        v[0][slot_of[i]] = c1;
+        v[1][slot_of[i]] = c2;
+        v[2][slot_of[i]] = c3;
+    }
+    for (int k = 0; k < 3; k++) {
+        out[k] = or_slot_sum(v[k], nSlots);
+        free(v[k]);
+    }
+}
+
+/* cn from the two sums (calculatedampingcoefficient.m:90-112) */
+double or_damping_from_sums(double cn1, double cn2)
+{
+    double cn = 0;
+    if (cn2 != 0) {
+        if ((cn1 / cn2) > 0) cn = 2 * sqrt(cn1 / cn2);
+        else cn = 0;
+    }
+    if (cn > 2) cn = 1.9;
+    return cn;
+}
+
+/* Static reaction of calculatereactionforceFast.m:3-5 (F_z * cellVolume over a body's family) in the
+ * device's order: lane t of a 1024-lane block adds members t, t + 1024, .. in family order starting
+ * from 0, then the halving tree over the lanes. */
+double or_static_reaction(const double *F, const double *family, int nFam, double V, int nNodes)
+{
+    double lanes[1024];
+    for (int t = 0; t < 1024; t++) {
+        double c = 0;
+        for (int f = t; f < nFam; f += 1024) c = c + F[(int)family[f] - 1 + 2 * (size_t)nNodes] * V;
+        lanes[t] = c;
+    }
+    return or_tree_sum(lanes, 1024);
+}
+
+/* Equilibrium ratio of a displacement-controlled step, after calculateequilibriumstate.m:6-8 (the
+ * out-of-balance force over the applied load): sqrt(sum of F_d^2 over the free DOFs) * cellVolume
+ * divided by |static reaction|; +infinity while the reaction is 0. */
+double or_equilibrium_ratio(double res2, double V, double reaction)
+{
+    double a = fabs(reaction);
+    return a > 0 ? (sqrt(res2) * V) / a : INFINITY;
 }
 
 /* ------------------------------------------------------------------------------------------------
@@ -421,6 +509,15 @@ void or_calculatedamageenergy(const double *c, const double *s0, const double *s
     }
 }
 
+/* Load-point displacement of step i under a staircase of increments: increment * ceil(i / ipi), the
+ * penetrator moved by `increment` every ipi ADR iterations (the "5e-6 m per increment" protocol;
+ * the legacy prescribed displacement of solvers/scripts/dynamic/timeintegration.m:40 applies one
+ * such increment). */
+double or_staircase(long i, long ipi, double increment)
+{
+    return increment * (double)((i + ipi - 1) / ipi);
+}
+
 /* ================================================================================================
  * One explicit step in the reference driver's order
  * (solvers/scripts/dynamic/dynamicsolverdisplacementcontrolled.m:71-121), ADR variant from
@@ -448,10 +545,29 @@ typedef struct or_state {
     /* scratch */
     double *l, *dX, *dY, *dZ, *s, *bsf, *fX, *fY, *fZ, *sp, *sXs, *sYs, *sZs;
     double *s0_full, *sc_full; /* expanded per-bond copies for the exponential routine */
+    /* load schedule, the load-point displacement of step i:
+     *   0  smoothstepdata(i, 1, nTimeSteps, 0, applied)   (dynamicsolverdisplacementcontrolled.m:76)
+     *   1  or_staircase(i, ipi, increment)
+     *   2  equilibrium staircase: increment * inc, where inc (from 1) advances after the step at
+     *      whose end the equilibrium ratio (or_equilibrium_ratio of this step's forces and static
+     *      reaction) is below eq_tol after at least ipi_min iterations at inc, or after ipi
+     *      iterations at inc (ADR only) */
+    int schedule, nTimeSteps, ipi, ipi_min;
+    double applied, increment, eq_tol;
+    /* order of the ADR sums and of the static reaction: slot_of NULL = the reference's sequential
+     * order; else node i's terms sit at slot slot_of[i] of nSlots and are summed by or_slot_sum */
+    const int *slot_of;
+    long nSlots;
+    /* load state (in/out): the current increment and the iterations done at it (schedule 2) */
+    long inc, inc_iters;
+    /* records of the last step (out): displacement, static reaction of body 0, equilibrium ratio,
+     * damping coefficient, 1 when the step ended a load increment (schedules 1 and 2) */
+    double last_disp, last_static, last_ratio, last_cn;
+    int last_inc_end;
 } or_state;
 
-/* returns the reaction of body 0; disp is the total load-point displacement for this step */
-double or_step(or_state *S, int iTimeStep, double disp, double *fz_bodies)
+/* returns the contact reaction of body 0 (calculatecontactforce.m:121) */
+double or_step(or_state *S, int iTimeStep, double *fz_bodies)
 {
     int nN = S->nNodes, nB = S->nBonds;
     or_calculatedeformedlength(S->x, S->BONDLIST, S->L0, nB, nN, S->l, S->dX, S->dY, S->dZ, S->s);
@@ -483,6 +599,8 @@ double or_step(or_state *S, int iTimeStep, double disp, double *fz_bodies)
         or_calculatenodalforces_openmp(nB, nN, S->BONDLIST, S->F, S->fX, S->fY, S->fZ, S->sXs, S->sYs, S->sZs,
                                        S->nodal_threads);
     }
+    double res2 = 0;
+    S->last_cn = 0;
     if (S->integrator == 0) {
         or_timeintegrationeulercromer(S->a, S->F, S->DAMPING, S->v, S->rho, S->cons, S->DT, S->u, S->x, S->X, nN, 3);
     } else {
@@ -494,12 +612,34 @@ double or_step(or_state *S, int iTimeStep, double disp, double *fz_bodies)
         unsigned char *driven = (unsigned char *)calloc((size_t)nN, 1);
         for (int b = 0; b < S->nBodies; b++)
             for (int f = 0; f < S->nFams[b]; f++) driven[(int)S->families[b][f] - 1] = 1;
-        double cn = or_calculatedampingcoefficient_masked(S->F, S->mass, S->Fp, S->DT, S->v, S->u, nN, driven);
+        double cn;
+        if (S->slot_of) {
+            double sums[3];
+            or_dampingsums_device(S->F, S->mass, S->Fp, S->DT, S->v, S->u, nN, driven, S->cons, S->slot_of,
+                                  S->nSlots, sums);
+            cn = or_damping_from_sums(sums[0], sums[1]);
+            res2 = sums[2];
+        } else {
+            cn = or_calculatedampingcoefficient_masked(S->F, S->mass, S->Fp, S->DT, S->v, S->u, nN, driven);
+            for (int i = 0; i < nN; i++) {
+                if (driven[i]) continue;
+                for (int d = 0; d < 3; d++) {
+                    size_t q = i + (size_t)nN * d;
+                    if (S->cons[q] != 1) res2 = res2 + S->F[q] * S->F[q];
+                }
+            }
+        }
         free(driven);
+        S->last_cn = cn;
         /* v holds the previous half-step velocity on entry and the new one on exit */
         or_timeintegrationADR(iTimeStep, S->DT, S->v, S->F, S->mass, cn, S->v, S->u, S->cons, nN);
         for (size_t q = 0; q < (size_t)nN * 3; q++) S->x[q] = S->X[q] + S->u[q];
     }
+    double disp;
+    if (S->schedule == 0) disp = or_smoothstepdata((double)iTimeStep, 1, (double)S->nTimeSteps, 0, S->applied);
+    else if (S->schedule == 1) disp = or_staircase(iTimeStep, S->ipi, S->increment);
+    else disp = S->increment * (double)S->inc;
+    S->last_disp = disp;
     double r0 = 0;
     for (int b = 0; b < S->nBodies; b++) {
         double d = S->moves[b] ? disp : 0;
@@ -507,6 +647,28 @@ double or_step(or_state *S, int iTimeStep, double disp, double *fz_bodies)
                                              S->radii[b], d, S->X, S->x, S->u, S->v, S->DT, S->V, S->rho, nN);
         if (fz_bodies) fz_bodies[b] = fz;
         if (b == 0) r0 = fz;
+    }
+    /* static reaction of body 0 from this step's forces */
+    double rs = 0;
+    if (S->nBodies > 0) {
+        if (S->slot_of) {
+            rs = or_static_reaction(S->F, S->families[0], S->nFams[0], S->V, nN);
+        } else {
+            for (int f = 0; f < S->nFams[0]; f++) rs = rs + S->F[(int)S->families[0][f] - 1 + 2 * (size_t)nN] * S->V;
+        }
+    }
+    S->last_static = rs;
+    S->last_ratio = or_equilibrium_ratio(res2, S->V, rs);
+    S->last_inc_end = 0;
+    if (S->schedule == 1) {
+        S->last_inc_end = (iTimeStep % S->ipi) == 0;
+    } else if (S->schedule == 2) {
+        S->inc_iters += 1;
+        if ((S->inc_iters >= S->ipi_min && S->last_ratio < S->eq_tol) || S->inc_iters >= S->ipi) {
+            S->last_inc_end = 1;
+            S->inc += 1;
+            S->inc_iters = 0;
+        }
     }
     if (S->integrator == 1) memcpy(S->Fp, S->F, sizeof(double) * 3 * nN);
     return r0;

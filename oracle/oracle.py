@@ -152,11 +152,57 @@ def calculatedampingcoefficient_masked(F, m, Fp, DT, vp, u, skip) -> float:
              sk.ctypes.data_as(C.c_void_p))
 
 
-def dampingsums(F, m, Fp, DT, vp, u):
-    c1, c2 = C.c_double(), C.c_double()
-    lib().or_dampingsums(_p(F), _p(m), _p(Fp), C.c_double(DT), _p(vp), _p(u), C.c_int(F.shape[0]), C.byref(c1),
-                         C.byref(c2))
-    return c1.value, c2.value
+def tree_sum(v) -> float:
+    """The device's halving-tree sum of len(v) values (a power of two)."""
+    a = np.array(v, dtype=np.float64)
+    f = lib().or_tree_sum
+    f.restype = C.c_double
+    return f(_p(a), C.c_int(a.size))
+
+
+def slot_sum(by_slot) -> float:
+    """The device's sum of one value per node slot (blocks of 256 by the tree, then the block sums)."""
+    a = np.ascontiguousarray(by_slot, dtype=np.float64)
+    f = lib().or_slot_sum
+    f.restype = C.c_double
+    return f(_p(a), C.c_long(a.size))
+
+
+def dampingsums_device(F, m, Fp, DT, vp, u, skip, cons, slot_of, nSlots):
+    """(cn1, cn2, sum of F^2 over free DOFs) in the device's order."""
+    out = np.zeros(3)
+    sk = np.ascontiguousarray(skip, dtype=np.uint8)
+    so = np.ascontiguousarray(slot_of, dtype=np.int32)
+    lib().or_dampingsums_device(_p(F), _p(m), _p(Fp), C.c_double(DT), _p(vp), _p(u), C.c_int(F.shape[0]),
+                                sk.ctypes.data_as(C.c_void_p), _p(_f(cons)) if cons is not None else None,
+                                so.ctypes.data_as(C.c_void_p), C.c_long(nSlots), _p(out))
+    return tuple(float(x) for x in out)
+
+
+def damping_from_sums(cn1, cn2) -> float:
+    f = lib().or_damping_from_sums
+    f.restype = C.c_double
+    return f(C.c_double(cn1), C.c_double(cn2))
+
+
+def static_reaction(F, family, V) -> float:
+    """Static reaction (calculatereactionforceFast.m:3-5) in the device's 1024-lane order."""
+    f = lib().or_static_reaction
+    f.restype = C.c_double
+    fam = _f(family)
+    return f(_p(F), _p(fam), C.c_int(fam.size), C.c_double(V), C.c_int(F.shape[0]))
+
+
+def staircase(i, ipi, increment) -> float:
+    f = lib().or_staircase
+    f.restype = C.c_double
+    return f(C.c_long(i), C.c_long(ipi), C.c_double(increment))
+
+
+def equilibrium_ratio(res2, V, reaction) -> float:
+    f = lib().or_equilibrium_ratio
+    f.restype = C.c_double
+    return f(C.c_double(res2), C.c_double(V), C.c_double(reaction))
 
 
 def timeintegrationADR(it, DT, vf, F, m, cn, vp, u, cons):
@@ -212,13 +258,33 @@ class _State(C.Structure):
         ("x", _d), ("u", _d), ("v", _d), ("a", _d), ("F", _d), ("Fp", _d), ("H", _d), ("flag", _d), ("fail", _d),
         ("l", _d), ("dX", _d), ("dY", _d), ("dZ", _d), ("s", _d), ("bsf", _d), ("fX", _d), ("fY", _d), ("fZ", _d),
         ("sp", _d), ("sXs", _d), ("sYs", _d), ("sZs", _d), ("s0_full", _d), ("sc_full", _d),
+        ("schedule", C.c_int), ("nTimeSteps", C.c_int), ("ipi", C.c_int), ("ipi_min", C.c_int),
+        ("applied", C.c_double), ("increment", C.c_double), ("eq_tol", C.c_double),
+        ("slot_of", C.POINTER(C.c_int)), ("nSlots", C.c_long),
+        ("inc", C.c_long), ("inc_iters", C.c_long),
+        ("last_disp", C.c_double), ("last_static", C.c_double), ("last_ratio", C.c_double),
+        ("last_cn", C.c_double), ("last_inc_end", C.c_int),
     ]
+
+
+def stencil_slots(case, align: int = 2048):
+    """The device's node slots in its stencil layout: the x-fastest index of the node's site in the
+    lattice box (coordinates are DX * k), node count padded to a multiple of `align`.  Returns
+    (slot_of, nSlots)."""
+    k = np.rint(np.asarray(case.X) / case.DX).astype(np.int64)
+    k -= k.min(axis=0)
+    dims = k.max(axis=0) + 1
+    slot = k[:, 0] + dims[0] * (k[:, 1] + dims[1] * k[:, 2])
+    n = case.X.shape[0]
+    return slot.astype(np.int32), int((n + align - 1) // align * align)
 
 
 class OracleRun:
     """The reference step sequence on a beam case (any object with the BeamCase fields)."""
 
-    def __init__(self, case, nodal_threads: int = 0):
+    def __init__(self, case, nodal_threads: int = 0, device_order: bool = False):
+        """`device_order`: the ADR sums and the static reaction in the order of the device's stencil
+        layout (or_slot_sum over stencil_slots(case)) instead of the reference's sequential order."""
         lib()
         if C.sizeof(_State) != lib().or_sizeof_state():
             raise RuntimeError("oracle state layout mismatch")
@@ -279,22 +345,40 @@ class OracleRun:
                      "fY", "fZ", "sp"):
             setattr(S, name, _p(getattr(self, name)))
         S.sXs, S.sYs, S.sZs = _p(self.sX), _p(self.sY), _p(self.sZ)
+        S.schedule, S.nTimeSteps = case.schedule, case.nTimeSteps
+        S.ipi, S.ipi_min = case.itersPerIncrement, getattr(case, "itersPerIncrementMin", 0)
+        S.applied, S.increment, S.eq_tol = case.appliedDisplacement, case.increment, getattr(case, "eqTol", 0.0)
+        S.inc, S.inc_iters = 1, 0
+        if device_order:
+            self._slot_of, S.nSlots = stencil_slots(case)
+            S.slot_of = self._slot_of.ctypes.data_as(C.POINTER(C.c_int))
         self.S = S
         self.step_count = 0
         self.reactions = []
+        self.records = {k: [] for k in ("disp", "reaction_static", "ratio", "cn", "inc_end")}
         f = lib().or_step
         f.restype = C.c_double
         self._step = f
 
     def step(self, n: int = 1):
+        """n steps; the load-point displacement follows the case's schedule inside or_step."""
         fz = (C.c_double * max(len(self.case.bodies), 1))()
+        S, rec = self.S, self.records
         for _ in range(n):
             i = self.step_count + 1
-            disp = self.case.displacement(i)
-            r = self._step(C.byref(self.S), C.c_int(i), C.c_double(disp), fz)
+            r = self._step(C.byref(S), C.c_int(i), fz)
             self.reactions.append(r)
+            rec["disp"].append(S.last_disp)
+            rec["reaction_static"].append(S.last_static)
+            rec["ratio"].append(S.last_ratio)
+            rec["cn"].append(S.last_cn)
+            rec["inc_end"].append(S.last_inc_end)
             self.step_count = i
         return self.reactions[-1] if self.reactions else 0.0
+
+    def load_state(self):
+        """(current increment, iterations done at it) of the equilibrium staircase."""
+        return int(self.S.inc), int(self.S.inc_iters)
 
     def step_omp(self, n: int, threads: int):
         """Timed baseline: the reference's CPU passes with OpenMP (Euler-Cromer order)."""

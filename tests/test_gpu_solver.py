@@ -6,9 +6,9 @@ Bars
     reference order, step after step;
   * exponential law: exp() differs from glibc by ulps, so 1e-10 relative (north-star tolerance for
     nodal forces) with identical flags on these short runs;
-  * ADR: the damping coefficient is a tree sum on the device and a sequential sum in the reference,
-    so one step from identical state agrees to 1e-12; free-running, the converged state at the end of
-    each load increment agrees to the ADR convergence floor (1e-4);
+  * ADR (stencil layout): bit-identical to the oracle run with the device's summation order for the
+    damping sums and the static reaction (OracleRun(device_order=True)), per step and free-running,
+    including the equilibrium staircase's increment ends and the 20,000-node beam;
   * FP32-mixed: stated error against FP64 on every result, flips of near-threshold bonds counted.
 """
 import numpy as np
@@ -188,52 +188,116 @@ def adr_case(ipi=400):
     return case
 
 
+HIST_EXACT = ("reaction", "reaction_static", "deflection", "cmod", "disp", "ratio", "cn")
+
+
+def compare_history(h, run, n, case):
+    """The last n rows of the device history against the oracle's records, bit for bit."""
+    assert len(h["reaction_static"]) == n
+    ref = case.referenceNode - 1
+    assert same(h["reaction"], run.reactions[-n:]), "reaction"
+    for k in ("reaction_static", "disp", "ratio", "cn"):
+        assert same(h[k], run.records[k][-n:]), k
+    assert np.array_equal(h["inc_end"], np.array(run.records["inc_end"][-n:], dtype=bool)), "inc_end"
+    assert same(h["deflection"][-1:], [run.u[ref, 2]])
+    c1, c2 = (i - 1 for i in case.cmod)
+    assert same(h["cmod"][-1:], [run.u[c2, 0] - run.u[c1, 0]])
+
+
 def test_fp64_adr_per_step(gpu, oracle):
-    """ADR: from the oracle's exact state, one device step (force, damping coefficient as a tree sum
-    over the nodes no contact body drives, leapfrog update, constraints, contact) matches the
-    oracle's step within 1e-12; the damping coefficient itself within 1e-12."""
+    """ADR: from the oracle's exact state, one device step (force, damping coefficient over the nodes
+    no contact body drives, leapfrog update, constraints, contact, static reaction) equals the
+    oracle's step bit for bit.  The oracle takes the damping sums and the static reaction in the
+    device's summation order (or_slot_sum, or_static_reaction), which is what makes ADR comparable
+    bit for bit."""
     case = adr_case()
     print(f"case: ADR bilinear per step, {case.nBonds} bonds", flush=True)
-    run = oracle.OracleRun(case)
-    driven = np.zeros(case.nNodes, np.uint8)
-    for b in case.bodies:
-        driven[b.family.astype(int) - 1] = 1
+    run = oracle.OracleRun(case, device_order=True)
     with Solver(case, _lib.FP64) as sol:
+        assert sol.layout() == "stencil"
         for k in (0, 1, 40, 399, 400, 401, 700):
             run.step(k - run.step_count)
             st = oracle_state(run)
             sol.restore(k, **st)
+            sol.history()
             sol.step(1)
             run.step(1)
-            compare_state(sol, run, exact=False, rtol=1e-12)
-            cn = oracle.calculatedampingcoefficient_masked(run.F, case.mass, st["F"], case.DT, st["v"], st["u"], driven)
-            got = sol.observables()["cn"]
-            assert got == pytest.approx(cn, rel=1e-12, abs=1e-14), (k, got, cn)
+            compare_state(sol, run, exact=True)
+            compare_history(sol.history(), run, 1, case)
+            assert same([sol.observables()["cn"]], run.records["cn"][-1:]), k
 
 
 def test_fp64_adr_increments(gpu, oracle):
-    """ADR free-running over three converged load increments: the static reaction, the deflection and
-    the softening flags at the end of each increment.  Near convergence the damping estimate is
-    noise-driven (velocities ~1e-11), so trajectories that differ by rounding agree to the ADR
-    convergence floor, not bit for bit: 1e-4 relative."""
+    """ADR free-running over three load increments of 400 iterations: the whole state after every
+    increment and every per-step history row (reactions, deflection, CMOD, displacement, equilibrium
+    ratio, damping coefficient, increment ends) bit for bit."""
     ipi = 400
     case = adr_case(ipi)
     print(f"case: ADR bilinear 3 increments, {case.nBonds} bonds", flush=True)
-    run = oracle.OracleRun(case)
-    fam = case.bodies[0].family.astype(int) - 1
-    ref = case.referenceNode - 1
+    run = oracle.OracleRun(case, device_order=True)
     with Solver(case, _lib.FP64) as sol:
         for inc in range(3):
             sol.step(ipi)
             run.step(ipi)
+            compare_state(sol, run, exact=True)
             h = sol.history()
-            R = (run.F[fam, 2] * case.cellVolume).sum()
-            assert R > 100 * (inc + 1)                       # loaded: the supports carry the beam
-            assert h["reaction_static"][-1] == pytest.approx(R, rel=1e-4)
-            assert h["deflection"][-1] == pytest.approx(run.u[ref, 2], rel=1e-4)
-            assert abs(int(sol.bonds()[1].sum()) - int(run.flag.sum())) <= 2
+            compare_history(h, run, ipi, case)
+            assert h["inc_end"][-1] and h["reaction_static"][-1] > 100 * (inc + 1)
         obs = sol.observables()
         assert obs["step"] == 3 * ipi
+        assert obs["n_softening"] == int(run.flag.sum())
+
+
+def test_fp64_adr_equilibrium_staircase(gpu, oracle):
+    """The equilibrium staircase: each increment ends once the equilibrium ratio falls below 1e-4
+    (after at least 20 iterations, at most 400).  The increment ends, the displacements and the
+    whole trajectory equal the oracle's bit for bit; restoring the load state resumes it."""
+    case = adr_case(400)
+    case.schedule, case.eqTol, case.itersPerIncrementMin = 2, 1e-4, 20
+    print(f"case: ADR equilibrium staircase, {case.nBonds} bonds", flush=True)
+    run = oracle.OracleRun(case, device_order=True)
+    with Solver(case, _lib.FP64) as sol:
+        for blk in range(3):
+            sol.step(500)
+            run.step(500)
+            compare_state(sol, run, exact=True)
+            compare_history(sol.history(), run, 500, case)
+            assert sol.load_state() == run.load_state()
+        ends = np.nonzero(run.records["inc_end"])[0]
+        assert len(ends) >= 4
+        assert min(np.diff(ends)) < 400           # some increments converged before the cap
+        ck = sol.checkpoint()
+        sol.step(200)
+        run.step(200)
+        h_ref = sol.history()
+        sol.restore(ck["step"], **{k: v for k, v in ck.items() if k != "step"})
+        sol.step(200)
+        h = sol.history()
+        for k in HIST_EXACT:
+            assert same(h[k], h_ref[k]), k
+        compare_state(sol, run, exact=True)
+
+
+def test_config0_adr_bit_exact(gpu, oracle):
+    """The 20,000-node beam (100 x 10 x 20 lattice, dx = 5 mm, 1,005,544 bonds, bilinear law with the
+    seeded +-1% s0 perturbation, sc = 2e-3) under ADR on the staircase of 5 um per 1000 iterations:
+    2,500 iterations (two and a half load increments).  Nodal forces, displacements, velocities,
+    softening factors, flags and the load-deflection record equal the oracle's bit for bit; the
+    asserted bar is the 1e-10 relative tolerance the comparison must meet at least, checked first."""
+    case = beam.case_bilinear_dx5()
+    assert case.integrator == _lib.ADR and case.itersPerIncrement == 1000
+    print(f"case: config-0 beam ADR, {case.nNodes} nodes {case.nBonds} bonds", flush=True)
+    run = oracle.OracleRun(case, device_order=True)
+    with Solver(case, _lib.FP64) as sol:
+        for blk in range(5):
+            sol.step(500)
+            run.step(500)
+            compare_state(sol, run, exact=False, rtol=1e-10)
+            compare_state(sol, run, exact=True)
+            compare_history(sol.history(), run, 500, case)
+            print(f"  {run.step_count} steps: load {run.records['reaction_static'][-1]:.3f} N, "
+                  f"{int(run.flag.sum())} softening bonds", flush=True)
+        assert run.records["reaction_static"][-1] > 0
 
 
 def test_full_beam_dx5_bit_exact(gpu, oracle):

@@ -272,8 +272,14 @@ def test_adr_update_recurrence(oracle):
 
 
 def test_mass_vector(oracle):
-    m = oracle.buildmassmatrix(0.5, 8.0, 2.0, np.array([10.0]))
-    assert m[0] == 0.25 * 4.0 * (10 * 0.25) * (8.0 / 0.5)
+    """buildmassmatrix.m:32-40: m = 1/4 DT^2 (nFAM DX^2)(c / DX).  With DX = 0.5, c = 8, DT = 2 and
+    nFAM = 10 that is 0.25 * 4 * 2.5 * 16 = 40 (DX in place of DX^2 would give 80, DT in place of
+    DT^2 gives 20); the mass is linear in nFAM, scales with DX at fixed c and with DT^2."""
+    m = oracle.buildmassmatrix(0.5, 8.0, 2.0, np.array([10.0, 3.0]))
+    assert m[0] == 40.0
+    assert m[1] == 12.0
+    assert oracle.buildmassmatrix(1.0, 8.0, 2.0, np.array([10.0]))[0] == 80.0
+    assert oracle.buildmassmatrix(0.5, 8.0, 6.0, np.array([10.0]))[0] == 360.0
 
 
 # ---------------------------------------------------------------- contact
@@ -347,7 +353,7 @@ def test_step_order_single_bond(oracle):
     c.cellVolume, c.DT, c.DAMPING = 1.0, 0.5, 0.0
     c.law, c.integrator = 1, 0
     c.bodies = []
-    c.displacement = lambda i: 0.0
+    c.schedule, c.nTimeSteps, c.itersPerIncrement, c.increment, c.appliedDisplacement = 1, 2, 1, 0.0, 0.0
     run = oracle.OracleRun(c)
     run.x[1, 0] = 1.25                                  # pre-stretched: s = 0.25 (< s0, elastic)
     run.step(1)
@@ -355,3 +361,155 @@ def test_step_order_single_bond(oracle):
     assert run.F[0, 0] == 0.5 and run.F[1, 0] == -0.5
     assert run.v[1, 0] == -0.25 and run.u[1, 0] == -0.125 and run.x[1, 0] == 0.875
     assert run.u[0, 0] == 0.0
+
+
+# ---------------------------------------------------------------- the device's summation orders
+def test_tree_sum_hand_values(oracle):
+    """The halving tree adds v[t] + v[t + n/2] first.  2^53 + 1 rounds back to 2^53 (a tie, to even),
+    so left to right [2^53, 1, 1, 0] sums to 2^53, while the tree pairs (2^53 + 1) and (1 + 0) and
+    gives 2^53 again; put the ones in the pair v[1], v[3] and the tree adds them first: 2^53 + 2."""
+    b = 2.0 ** 53
+    assert oracle.tree_sum([b, 1.0, 1.0, 0.0]) == b
+    assert oracle.tree_sum([b, 1.0, 0.0, 1.0]) == b + 2
+    assert ((b + 1.0) + 0.0) + 1.0 == b                      # the sequential sum of the latter
+    assert oracle.tree_sum([3.0]) == 3.0
+    v = np.arange(1024, dtype=np.float64)
+    assert oracle.tree_sum(v) == 1023 * 1024 / 2             # exact in any order
+
+
+def test_slot_sum_order(oracle):
+    """Blocks of 256 slots by the tree, then lane t of 256 adds the block sums t, t + 256, ... in
+    order: with 257 blocks, block 0 and block 256 meet in lane 0 before the lane tree."""
+    b = 2.0 ** 53
+    v = np.zeros(257 * 256)
+    v[0] = b                 # block 0
+    v[1] = 1.0               # block 0, pairs with slot 129 at the first tree level
+    v[129] = 1.0
+    assert oracle.slot_sum(v) == b + 2
+    w = np.zeros(257 * 256)
+    w[0] = b                 # block 0 -> lane 0
+    w[256 * 256] = 1.0       # block 256 -> lane 0, added to block 0's sum: a tie, stays 2^53
+    w[256 * 1] = 1.0         # block 1 -> lane 1; the lane tree adds lanes 0 and 1 last... (b + 1)
+    assert oracle.slot_sum(w) == b
+    w2 = w.copy()
+    w2[256 * 256] = 0.0
+    w2[256 * 129] = 1.0      # block 129 -> lane 129, paired with lane 1 at the first lane level
+    assert oracle.slot_sum(w2) == b + 2
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(300 * 256) * 10.0 ** rng.integers(-8, 8, 300 * 256)
+    assert abs(oracle.slot_sum(x) - math.fsum(x)) <= 300 * 256 * 2.0 ** -52 * np.abs(x).sum()
+
+
+def _adr_arrays(n, seed):
+    rng = np.random.default_rng(seed)
+    F = np.asfortranarray(rng.standard_normal((n, 3)))
+    Fp = np.asfortranarray(rng.standard_normal((n, 3)))
+    vp = np.asfortranarray(rng.standard_normal((n, 3)))
+    vp[rng.random((n, 3)) < 0.1] = 0.0
+    u = np.asfortranarray(rng.standard_normal((n, 3)) * 1e-3)
+    m = rng.uniform(1.0, 2.0, n)
+    return F, m, Fp, vp, u
+
+
+def test_device_damping_sums(oracle):
+    """The device-order ADR sums: for one node (the only nonzero slot) they equal the reference's
+    sequential quotient bit for bit; over many nodes they agree with a correctly rounded sum of the
+    same per-node terms; skipped nodes and constrained DOFs (the residual) are left out."""
+    n = 3000
+    F, m, Fp, vp, u = _adr_arrays(n, 1)
+    DT = 1.0
+    one = np.zeros(n, np.uint8) + 1
+    one[17] = 0
+    slots = np.arange(n, dtype=np.int32)
+    c1, c2, r = oracle.dampingsums_device(F, m, Fp, DT, vp, u, one, None, slots, 4096)
+    assert oracle.damping_from_sums(c1, c2) == oracle.calculatedampingcoefficient_masked(F, m, Fp, DT, vp, u, one)
+    assert r == F[17, 0] ** 2 + F[17, 1] ** 2 + F[17, 2] ** 2
+    none = np.zeros(n, np.uint8)
+    cons = np.zeros((n, 3), order="F")
+    cons[::7, 2] = 1
+    c1, c2, r = oracle.dampingsums_device(F, m, Fp, DT, vp, u, none, cons, slots, 4096)
+    t1 = [-(u[i, d] ** 2) * ((F[i, d] / m[i] - Fp[i, d] / m[i]) / (DT * vp[i, d]))
+          for i in range(n) for d in range(3) if vp[i, d] != 0]
+    t2 = [u[i, d] ** 2 for i in range(n) for d in range(3)]
+    t3 = [F[i, d] ** 2 for i in range(n) for d in range(3) if cons[i, d] != 1]
+    for got, terms in ((c1, t1), (c2, t2), (r, t3)):
+        assert abs(got - math.fsum(terms)) <= 8 * len(terms) * 2.0 ** -53 * math.fsum(abs(t) for t in terms)
+    # a permuted slot map changes nothing but the order
+    perm = np.random.default_rng(2).permutation(4096)[:n].astype(np.int32)
+    c1p, c2p, rp = oracle.dampingsums_device(F, m, Fp, DT, vp, u, none, cons, perm, 4096)
+    assert c2p == pytest.approx(c2, rel=1e-13) and rp == pytest.approx(r, rel=1e-13)
+
+
+def test_static_reaction_order(oracle):
+    """1024 lanes, member t + 1024 j added to lane t in order, then the tree: with 2^53 at member 0
+    and ones at members 1 and 513 (lanes paired at the first level), 2^53 + 2 per unit volume."""
+    b = 2.0 ** 53
+    n = 3000
+    F = np.zeros((n, 3), order="F")
+    F[0, 2], F[1, 2], F[513, 2] = b, 1.0, 1.0
+    fam = np.arange(1, n + 1, dtype=np.float64)
+    assert oracle.static_reaction(F, fam, 1.0) == b + 2
+    F2 = np.zeros((n, 3), order="F")
+    F2[0, 2], F2[1024, 2], F2[1, 2] = b, 1.0, 1.0      # member 1024 joins lane 0 first: a tie
+    assert oracle.static_reaction(F2, fam, 1.0) == b
+    rng = np.random.default_rng(3)
+    F3 = np.asfortranarray(rng.standard_normal((n, 3)))
+    sub = np.sort(rng.choice(n, 700, replace=False) + 1).astype(np.float64)
+    exact = math.fsum(F3[sub.astype(int) - 1, 2] * 0.125)
+    assert oracle.static_reaction(F3, sub, 0.125) == pytest.approx(exact, rel=1e-13)
+
+
+def test_staircase_and_equilibrium_ratio(oracle):
+    assert oracle.staircase(1, 1000, -5e-6) == -5e-6
+    assert oracle.staircase(1000, 1000, -5e-6) == -5e-6
+    assert oracle.staircase(1001, 1000, -5e-6) == -1e-5
+    assert oracle.staircase(7, 3, 2.0) == 6.0
+    assert oracle.equilibrium_ratio(16.0, 0.5, -4.0) == 0.5        # sqrt(16) * 0.5 / |-4|
+    assert oracle.equilibrium_ratio(1.0, 1.0, 0.0) == math.inf
+
+
+def _adr_beam(schedule, ipi, **kw):
+    from bb_pd_amd import _lib, beam
+    DX = 5e-3
+    case = beam.build_beam("t", 25, 4, 10, DX, law=_lib.LAW_BILINEAR, s0=1e-4, s0_perturb=0.01, sc=2e-3,
+                           integrator=_lib.ADR, iters_per_increment=ipi, increment=-4e-6,
+                           supports_x=(3 * DX, 23 * DX), penetrator_x=13 * DX, schedule=schedule, **kw)
+    return case
+
+
+def test_equilibrium_staircase_logic(oracle):
+    """The equilibrium staircase with tolerance 0 advances every ipi iterations: the same
+    displacements, and so the same trajectory bit for bit, as the plain staircase, with the
+    increment ends at the same steps.  With a tolerance of +inf every state passes, and it advances
+    every ipi_min iterations."""
+    a = oracle.OracleRun(_adr_beam(1, 7), device_order=True)
+    b = oracle.OracleRun(_adr_beam(2, 7, eq_tol=0.0), device_order=True)
+    a.step(40)
+    b.step(40)
+    assert a.records["disp"] == b.records["disp"]
+    assert a.records["inc_end"] == b.records["inc_end"] == [int(i % 7 == 0) for i in range(1, 41)]
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v)
+    assert b.load_state() == (6, 40 - 35)
+    c = oracle.OracleRun(_adr_beam(2, 50, eq_tol=math.inf, iters_per_increment_min=3), device_order=True)
+    c.step(12)
+    r = c.records
+    assert r["reaction_static"][0] == 0.0 and r["ratio"][0] == math.inf   # u = 0 at step 1
+    ends = [i + 1 for i, e in enumerate(r["inc_end"]) if e]
+    assert ends == [3, 6, 9, 12]
+    assert r["disp"] == [-4e-6 * (1 + (i - 1) // 3) for i in range(1, 13)]
+
+
+def test_device_order_trajectory_close_to_reference_order(oracle):
+    """The device summation order changes only the rounding of the ADR sums and of the static
+    reaction: over the first 5 ADR steps the two orders' states agree to rounding-level
+    differences (later, ADR's damping estimate amplifies rounding differences, which is why a
+    bitwise comparison with the device needs the device's order)."""
+    case = _adr_beam(1, 20)
+    a = oracle.OracleRun(case)
+    b = oracle.OracleRun(case, device_order=True)
+    a.step(5)
+    b.step(5)
+    assert np.abs(a.u - b.u).max() <= 1e-12 * np.abs(a.u).max()
+    ra, rb = np.array(a.records["reaction_static"]), np.array(b.records["reaction_static"])
+    assert np.abs(ra - rb).max() <= 1e-12 * np.abs(ra).max()
+    assert np.array(a.records["cn"]) == pytest.approx(np.array(b.records["cn"]), rel=1e-12, abs=1e-300)
