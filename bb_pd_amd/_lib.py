@@ -120,6 +120,7 @@ SIGNATURES = {
     "bbpd_solver_set_load_state": (_I, [_V, _L, _L]),
     "bbpd_solver_info": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_layout": (C.c_char_p, [_V]),
+    "bbpd_solver_bond_bytes": (_I, [_V, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L)]),
     "bbpd_solver_autotune": (_I, [_V, _I, _P, C.POINTER(_I)]),
     "bbpd_solver_damage": (_I, [_V, _P, C.POINTER(_D)]),
     "bbpd_batch_damage": (_I, [_V, _I, _P, C.POINTER(_D)]),

@@ -124,6 +124,16 @@ def lattice(nx, ny, nz, DX, horizon_factor=3.015, notch=None, E=37e9):
     return X, BL, L0, vcf, c, nFAM
 
 
+def _ec_dt(rho, DX, horizon, c0):
+    """Gregoire_beam_tests.m:244: the Euler-Cromer stable time step with a 0.8 safety factor."""
+    return (0.8 * math.sqrt(2 * rho * DX / (math.pi * horizon ** 2 * DX * c0))) / 1
+
+
+def euler_cromer_dt(case: BeamCase) -> float:
+    """The Euler-Cromer time step of a beam case (whatever integrator it was built for)."""
+    return _ec_dt(float(case.DENSITY[0]), case.DX, case.horizon, case.c0)
+
+
 def build_penetrator(X: np.ndarray, cx: float, cz: float, R: float, search: float, moves: bool) -> Body:
     """buildpenetrator.m:10-24: family = nodes with undeformed x-z distance <= searchRadius."""
     d = np.sqrt((X[:, 0] - cx) ** 2 + (X[:, 2] - cz) ** 2)
@@ -195,7 +205,7 @@ def build_beam(name: str, nx: int, ny: int, nz: int, DX: float, *, law: int, hor
         cons[ids[len(ids) // 2], 1] = 1
     # time step
     if integrator == _lib.EULER_CROMER:
-        DT = (0.8 * math.sqrt(2 * rho * DX / (math.pi * horizon ** 2 * DX * c0))) / 1
+        DT = _ec_dt(rho, DX, horizon, c0)
         mass = None
     else:
         DT = 1.0

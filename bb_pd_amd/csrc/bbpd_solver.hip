@@ -3465,6 +3465,44 @@ extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *
     return BBPD_OK;
 }
 
+extern "C" int bbpd_solver_bond_bytes(bbpd_solver *s, int64_t *bytes, int64_t *offsets_set, int64_t *offsets_total)
+{
+    BBPD_REQUIRE(s && bytes, "bad arguments");
+    int64_t all = 0;
+    int rc = bbpd_solver_info(s, nullptr, &all, nullptr, nullptr);
+    if (rc) return rc;
+    if (!s->stencil) {
+        *bytes = all;
+        if (offsets_set) *offsets_set = 0;
+        if (offsets_total) *offsets_total = 0;
+        return BBPD_OK;
+    }
+    BBPD_HIP(hipSetDevice(s->device));
+    BBPD_HIP(hipStreamSynchronize(s->stream));
+    const bool mixed = s->precision == BBPD_PRECISION_MIXED;
+    const int64_t w = mixed ? 4 : 8;
+    const int64_t streams = (s->P.s0mode == PM_STREAM ? 1 : 0) + (s->P.scmode == PM_STREAM ? 1 : 0);
+    // the model of bbpd_solver_info counts a history word (and each streamed parameter) for every
+    // entry; the kernel reads a history word only where its slice's history word for the offset is
+    // set, 64 lanes at a time, and a streamed parameter only for an entry whose law runs, which such
+    // an offset holds once it has run
+    const int64_t nsl = (int64_t)s->nSlots / SLICE;
+    std::vector<unsigned long long> hw((size_t)nsl * 2);
+    BBPD_HIP(hipMemcpy(hw.data(), s->P.snz, sizeof(unsigned long long) * hw.size(), hipMemcpyDeviceToHost));
+    int64_t set = 0, total = 0;
+    for (size_t r = 0; r < s->reg.size(); r++) {
+        if (r >= s->active.size() || !s->active[r]) continue;
+        const Region &RG = s->reg[r];
+        const int64_t sl0 = RG.base / SLICE, sl1 = (RG.base + ((int64_t)RG.nN + SLICE - 1) / SLICE * SLICE) / SLICE;
+        for (int64_t q = sl0; q < sl1; q++) set += __builtin_popcountll(hw[2 * q]) + __builtin_popcountll(hw[2 * q + 1]);
+        total += (sl1 - sl0) * RG.noff;
+    }
+    *bytes = all - (w + streams * w) * s->nEntries + set * SLICE * (w + streams * w);
+    if (offsets_set) *offsets_set = set;
+    if (offsets_total) *offsets_total = total;
+    return BBPD_OK;
+}
+
 extern "C" const char *bbpd_solver_layout(bbpd_solver *s)
 {
     return s ? s->layout.c_str() : "";
@@ -3504,9 +3542,9 @@ extern "C" int bbpd_solver_autotune(bbpd_solver *s, int reps, double *ms, int *c
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (bestv != s->occ) s->version++; // captured graphs hold the old kernel
     s->occ = bestv;
     s->tuned = true;
-    s->version++; // captured graphs hold the old kernel
     if (choice) *choice = bestv;
     return BBPD_OK;
 }

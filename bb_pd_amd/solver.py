@@ -161,6 +161,12 @@ class Solver:
         return dict(device_bytes=v[0].value, bond_kernel_bytes=v[1].value, n_entries=v[2].value,
                     n_padded_entries=v[3].value)
 
+    def bond_bytes(self) -> dict:
+        """Algorithmic bytes of one bond-kernel launch at the current state (bbpd_solver_bond_bytes)."""
+        v = [C.c_int64() for _ in range(3)]
+        _lib.check(self._L.bbpd_solver_bond_bytes(self._h, *[C.byref(x) for x in v]), "bbpd_solver_bond_bytes")
+        return dict(bytes=v[0].value, offsets_set=v[1].value, offsets_total=v[2].value)
+
     def set_timing(self, on: bool):
         _lib.check(self._L.bbpd_solver_set_timing(self._h, 1 if on else 0), "bbpd_solver_set_timing")
 

@@ -273,6 +273,13 @@ int bbpd_solver_damage(bbpd_solver *s, double *damage, double *energy);
 /* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */
 int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel_bytes, int64_t *n_entries,
                      int64_t *n_padded_entries);
+/* Algorithmic HBM bytes of one bond-kernel launch at the current state.  bbpd_solver_info counts a
+ * history word for every entry; the stencil layout's kernel reads the history of an offset only in
+ * the slices (64 nodes) whose history word for it is set, so here the history (and any per-bond
+ * parameter stream, read only by entries whose law runs) counts 64 entries per set (slice, offset)
+ * pair: offsets_set of offsets_total such pairs.  The record layout returns the bbpd_solver_info
+ * figure (and 0 pairs).  No counterpart in the reference. */
+int bbpd_solver_bond_bytes(bbpd_solver *s, int64_t *bytes, int64_t *offsets_set, int64_t *offsets_total);
 /* the bond-table layout chosen at set-up: "stencil" (full lattice box, rows in lattice-offset
  * order: neighbours implicit, per entry only the history) or "records (<why not stencil>)".  The
  * string lives as long as the solver.  Default: stencil where it applies, for both forms;
