@@ -195,7 +195,11 @@ def test_adr_routines(gpu, oracle, case):
     m = rng.uniform(1, 2, nN)
     cn_o = oracle.calculatedampingcoefficient(F, m, Fp, 1.0, vp, u)
     cn_d = mex.calculatedampingcoefficient(D(F), D(m), D(Fp), 1.0, D(vp), D(u))
-    assert cn_d == pytest.approx(cn_o, rel=1e-12)
+    assert cn_d == pytest.approx(cn_o, rel=1e-12)          # the reference's sequential order
+    # the device's order (nodes in blocks of 256 by the tree, then the block sums): bit for bit
+    c1, c2, _ = oracle.dampingsums_device(F, m, Fp, 1.0, vp, u, np.zeros(nN, np.uint8), None,
+                                          np.arange(nN, dtype=np.int32), (nN + 255) // 256 * 256)
+    assert cn_d == oracle.damping_from_sums(c1, c2)
     cons = np.asfortranarray((rng.uniform(size=(nN, 3)) < 0.1).astype(float))
     for it in (1, 2):
         vo, uo = vp.copy(order="F"), u.copy(order="F")
