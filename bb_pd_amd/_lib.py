@@ -17,6 +17,7 @@ FP64, MIXED = 0, 1
 MAX_BODIES = 8
 AUTOTUNE_VARIANTS = 6  # BBPD_AUTOTUNE_VARIANTS
 HISTORY_COLUMNS = 8  # BBPD_HISTORY_COLUMNS
+NOALLOC_MAX_NODES = 16777216  # BBPD_NOALLOC_MAX_NODES
 
 _dp = C.POINTER(C.c_double)
 
@@ -100,6 +101,7 @@ SIGNATURES = {
     "bbpd_calculatebondsofteningfactor": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _I, _V]),
     "bbpd_calculatebsftrilinear": (_I, [_P, _D, _D, _D, _P, _P, _P, _I, _V]),
     "bbpd_calculatedampingcoefficient": (_I, [_P, _P, _P, _D, _P, _P, _I, _dp, _V]),
+    "bbpd_calculatedampingcoefficient_noalloc": (_I, [_P, _P, _P, _D, _P, _P, _I, _dp, _V]),
     "bbpd_timeintegrationADR": (_I, [_I, _D, _P, _P, _P, _D, _P, _P, _P, _I, _V]),
     "bbpd_calculatecontactforce": (_I, [_P, _I, _D, _D, _D, _D, _P, _P, _P, _P, _D, _D, _P, _I, _dp, _V]),
     "bbpd_beam_count": (_I, [C.POINTER(BeamSpec), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -625,6 +625,70 @@ extern "C" int bbpd_calculatedampingcoefficient(const double *F, const double *m
     return BBPD_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Allocation-free variant of bbpd_calculatedampingcoefficient.  Its block partials and result live
+// in `scratch`, a file-scope device array that each GPU holds for the life of the process.  In
+// host code the array's name stands for the symbol, not a device address: the address is looked
+// up once per device with hipGetSymbolAddress and only that address is handed to kernels and
+// copies.  Every call on a GPU shares that GPU's array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------------------------
+namespace {
+constexpr int NOALLOC_BLOCKS = (BBPD_NOALLOC_MAX_NODES + BLOCK - 1) / BLOCK;
+constexpr int NOALLOC_RESULT = 4; // doubles at the front: cn, cn1, cn2 (and one spare)
+} // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_RESULT + 2 * NOALLOC_BLOCKS];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64] = {nullptr};
+
+int scratch_ptr(double **out)
+{
+    int dev = 0;
+    BBPD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        bbpd::set_error("device ordinal out of range");
+        return BBPD_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return bbpd::hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return BBPD_OK;
+}
+
+} // namespace
+
+extern "C" int bbpd_calculatedampingcoefficient_noalloc(const double *F, const double *m, const double *Fp,
+                                                        double DT, const double *vp, const double *u, int nNodes,
+                                                        double *cn_host, void *stream)
+{
+    BBPD_REQUIRE(nNodes >= 0 && cn_host, "bad arguments");
+    BBPD_REQUIRE(nNodes <= BBPD_NOALLOC_MAX_NODES,
+                 "nNodes exceeds BBPD_NOALLOC_MAX_NODES (16777216) of the allocation-free entry point");
+    BBPD_REQUIRE(nNodes == 0 || (F && m && Fp && vp && u), "null array");
+    hipStream_t st = bbpd::as_stream(stream);
+    double *buf = nullptr;
+    int rc = scratch_ptr(&buf);
+    if (rc) return rc;
+    // every partial the final sum reads is written by the first kernel: nothing to clear
+    const unsigned nb = bbpd::grid_for(nNodes, BLOCK);
+    k_damping_partials<<<nb, BLOCK, 0, st>>>(F, m, Fp, DT, vp, u, nNodes, buf + NOALLOC_RESULT);
+    k_damping_final<<<1, BLOCK, 0, st>>>(buf + NOALLOC_RESULT, (int)nb, buf);
+    BBPD_HIP(hipGetLastError());
+    double r[3];
+    BBPD_HIP(hipMemcpyAsync(r, buf, sizeof(r), hipMemcpyDeviceToHost, st));
+    BBPD_HIP(hipStreamSynchronize(st));
+    *cn_host = r[0];
+    return BBPD_OK;
+}
+
 extern "C" int bbpd_timeintegrationADR(int it, double DT, double *vf, const double *F, const double *m, double cn,
                                        const double *vp, double *u, const double *cons, int nNodes, void *stream)
 {

@@ -112,6 +112,20 @@ int bbpd_calculatedampingcoefficient(const double *nodalForce, const double *mas
                                      const double *nodalVelocityPreviousHalf, const double *nodalDisplacement,
                                      int nNodes, double *cn_host, void *stream);
 
+/* No counterpart in the reference.  The same routine as bbpd_calculatedampingcoefficient (same
+ * arguments, kernels, summation order and result) for callers that cannot allocate device memory:
+ * tight serving loops and harnesses that forbid hipMalloc.  It calls no hipMalloc, hipFree or other
+ * allocating routine; its block partials live in a fixed device array of the library (about 1 MiB
+ * per GPU, held until the process ends), which bounds nNodes by BBPD_NOALLOC_MAX_NODES (larger
+ * inputs are rejected with BBPD_ERR_ARG).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other threads or streams) race. */
+#define BBPD_NOALLOC_MAX_NODES 16777216
+int bbpd_calculatedampingcoefficient_noalloc(const double *nodalForce, const double *massVector,
+                                             const double *nodalForcePrevious, double DT,
+                                             const double *nodalVelocityPreviousHalf,
+                                             const double *nodalDisplacement, int nNodes, double *cn_host,
+                                             void *stream);
+
 /* replaces solvers/functions/timeintegrationADR.m:1 (NOD = 3) */
 int bbpd_timeintegrationADR(int iTimeStep, double DT, double *nodalVelocityForwardHalf, const double *nodalForce,
                             const double *massVector, double cn, const double *nodalVelocityPreviousHalf,
