@@ -58,5 +58,50 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+MEX = PKG / "mex"
+MEX_OUT = OUT / "mex"
+C_FLAGS = ["-std=c11", "-O2", "-fPIC", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__"]
+
+
+def _rocm_include() -> str:
+    return str(Path(_hipcc()).resolve().parent.parent / "include")
+
+
+def _rocm_lib() -> str:
+    return str(Path(_hipcc()).resolve().parent.parent / "lib")
+
+
+def build_gateways(verbose: bool = False) -> list:
+    """Compile the MEX gateways (bb_pd_amd/mex/*Mex.c) against the project's mex.h stand-in: one
+    shared object per gateway, each exporting mexFunction, linked to libbbpd_hip.so and to
+    libmexshim.so (the stand-in's implementation).  Needs the library built first."""
+    MEX_OUT.mkdir(parents=True, exist_ok=True)
+    inc = ["-I", str(ROOT / "include"), "-I", str(MEX), "-I", _rocm_include()]
+    shim = MEX_OUT / "libmexshim.so"
+    cmd = ["gcc", *C_FLAGS, *inc, "-shared", "-Wl,-soname,libmexshim.so", str(MEX / "mexshim.c"), "-o", str(shim)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed on mexshim.c:\n{r.stdout}\n{r.stderr}")
+    out = []
+    srcs = sorted(MEX.glob("*Mex.c"))
+
+    def one(src):
+        so = MEX_OUT / (src.stem + ".so")
+        cmd = ["gcc", *C_FLAGS, *inc, "-shared", str(src), "-o", str(so), "-L", str(MEX_OUT), "-lmexshim",
+               "-L", str(OUT), "-lbbpd_hip", "-L", _rocm_lib(), "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN:$ORIGIN/..:" + _rocm_lib()]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"gcc failed on {src.name}:\n{r.stdout}\n{r.stderr}")
+        return so
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        out = list(ex.map(one, srcs))
+    if verbose:
+        print(f"built {len(out)} MEX gateways in {MEX_OUT}")
+    return out
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_gateways(verbose=True)
