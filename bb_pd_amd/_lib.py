@@ -126,6 +126,8 @@ SIGNATURES = {
     "bbpd_solver_autotune": (_I, [_V, _I, _P, C.POINTER(_I)]),
     "bbpd_solver_damage": (_I, [_V, _P, C.POINTER(_D)]),
     "bbpd_batch_damage": (_I, [_V, _I, _P, C.POINTER(_D)]),
+    "bbpd_solver_strain_energy": (_I, [_V, _P, C.POINTER(_D)]),
+    "bbpd_batch_strain_energy": (_I, [_V, _I, _P, C.POINTER(_D)]),
     "bbpd_batch_create": (_I, [_P, _I, _I, C.POINTER(_V)]),
     "bbpd_batch_admit": (_I, [_V, _P, _D, _D, C.POINTER(_I)]),
     "bbpd_batch_retire": (_I, [_V, _I]),

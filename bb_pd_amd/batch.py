@@ -144,6 +144,14 @@ class Batch:
                                              C.byref(e) if energy else None), "bbpd_batch_damage")
         return d, (e.value if energy else None)
 
+    def strain_energy(self, region: int):
+        """(strain energy density per node of the region's problem, total strain energy)."""
+        d = np.zeros(self.cases[region].nNodes)
+        e = C.c_double()
+        _lib.check(self._L.bbpd_batch_strain_energy(self._h, region, d.ctypes.data_as(C.c_void_p), C.byref(e)),
+                   "bbpd_batch_strain_energy")
+        return d, e.value
+
     def info(self) -> dict:
         v = [C.c_int64() for _ in range(4)]
         _lib.check(self._L.bbpd_solver_info(self._h, *[C.byref(x) for x in v]), "bbpd_solver_info")

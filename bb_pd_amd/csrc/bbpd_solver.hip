@@ -608,7 +608,8 @@ struct RowRecs {
 };
 
 // Per-offset tables are the same for every lane of a wavefront and no kernel writes them: read
-// through the constant address space, they become scalar loads into scalar registers.
+// through the constant address space, they become scalar loads into scalar registers.  Memory a
+// kernel writes (the history words) is not read this way.
 template <typename T>
 using sconst_ptr = const __attribute__((address_space(4))) T *;
 template <typename T>
@@ -845,12 +846,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const StOff *sst = P.sst + RG.sst0;
     const unsigned long long *mks = P.smaskT + eslice / SLICE;
     // the slice's history words (noff <= 128): an offset whose bit is clear holds +0 in every entry,
-    // which is taken without a load.  This wavefront alone writes them, and only the bit of an offset
-    // it has already loaded, so a scalar read of them at any point of the wavefront (the compiler may
-    // repeat it: the scalar path treats the words as constant) gives the same bits for the offsets
-    // still ahead.
+    // which is taken without a load.  This kernel sets bits of them (atomicOr below), so they are read
+    // once, here, with an ordinary global load (every lane the same address) made wavefront-uniform
+    // with readfirstlane, not through the constant address space; the bits set later are for offsets
+    // this wavefront has already passed.
     unsigned long long *hzp = P.snz + 2 * (n0 / SLICE);
-    const unsigned long long hz0 = sload(hzp, 0), hz1 = sload(hzp, 1);
+    auto uniform64 = [](unsigned long long w) {
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)w);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(w >> 32));
+        return ((unsigned long long)hi << 32) | lo;
+    };
+    const unsigned long long hz0 = uniform64(hzp[0]), hz1 = uniform64(hzp[1]);
     auto hbit = [&](int o) { return o < 128 && (((o < 64 ? hz0 : hz1) >> (o & 63)) & 1ull) != 0; };
     // offset o's loads; o runs to noff: the entry arrays end at least two slices' worth past the
     // last entry and the per-offset table ST_PAD records past the last offset
@@ -1467,8 +1473,12 @@ __device__ double damage_energy(double c, double s0, double sc, double L0, doubl
     return (numerator / denominator) * (V * V);
 }
 
+// Per node: the damage field (calculatedamage.m:34-45), the damage energy of the bonds at their lower
+// slot (block partials), and, when sed is given, the strain energy density of
+// calculatestrainenergy.m:10-24: each bond's micropotential 0.5 fail c s^2 L0 V vcf (left to right, s
+// from the current positions in FP64) added to both its nodes in BONDLIST order, halved.
 template <bool MIXED, int LAW>
-__global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg, double *epart)
+__global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg, double *epart, double *sed)
 {
     __shared__ double sh[BLOCK];
     const Region &RG = P.reg[ri];
@@ -1476,7 +1486,7 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
     const bool real = n - RG.base < RG.nN;
     const int cur = (int)(*P.tick & 1);
     const NodeRec *pos = P.pos[cur];
-    double e_acc = 0;
+    double e_acc = 0, mu_acc = 0;
     int cnt = 0, unbroken = 0;
     if (real) {
         const NodeRec me = pos[n];
@@ -1488,11 +1498,32 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
             Xn[1] = P.hy * (double)(int)((me.q >> LAT_BITS) & LAT_MASK);
             Xn[2] = P.hz * (double)(int)((me.q >> (2 * LAT_BITS)) & LAT_MASK);
         }
+        // deformed coordinates: the FP64 form stores them, the mixed form stores u (x = X + u)
+        const double xn0 = MIXED ? Xn[0] + me.x : me.x, xn1 = MIXED ? Xn[1] + me.y : me.y,
+                     xn2 = MIXED ? Xn[2] + me.z : me.z;
         auto bond = [&](double H, bool failed, int m, double c, double s0, double sc, double Xm0, double Xm1,
                         double Xm2) {
             const bool broken = failed || H == 1.0;
             cnt++;
             unbroken += broken ? 0 : 1;
+            if (sed) {
+                // buildhorizons.m:93-97, calculatedeformedlengthMex.c:40-43 and
+                // calculatevolumecorrectionfactors.m:20-32, then the micropotential of
+                // calculatestrainenergy.m:16 (BFMULTIPLIER = 1 for concrete, buildbonddata.m:56-57)
+                const double ex = Xn[0] - Xm0, ey = Xn[1] - Xm1, ez = Xn[2] - Xm2;
+                const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+                const NodeRec pm = pos[m];
+                const double dx = (MIXED ? Xm0 + pm.x : pm.x) - xn0, dy = (MIXED ? Xm1 + pm.y : pm.y) - xn1,
+                             dz = (MIXED ? Xm2 + pm.z : pm.z) - xn2;
+                const double l = sqrt(dx * dx + dy * dy + dz * dz);
+                const double st = (l - L0) / L0;
+                double vcf;
+                if (L0 <= P.hmr) vcf = 1;
+                else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
+                else vcf = 0;
+                const double fl = (LAW == BBPD_LAW_LINEAR && failed) ? 0.0 : 1.0;
+                mu_acc = mu_acc + 0.5 * fl * 1.0 * c * (st * st) * L0 * P.V * vcf;
+            }
             if (m > n && LAW != BBPD_LAW_LINEAR) { // each bond once, at its lower slot
                 const double ex = Xn[0] - Xm0, ey = Xn[1] - Xm1, ez = Xn[2] - Xm2;
                 const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
@@ -1572,7 +1603,8 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
                 bond(H, (w & FAILED_BIT) != 0, m, c, s0, sc, Xm[0], Xm[1], Xm[2]);
             }
         }
-        dmg[n - RG.base] = cnt > 0 ? 1 - ((double)unbroken / (double)cnt) : 0.0;
+        if (dmg) dmg[n - RG.base] = cnt > 0 ? 1 - ((double)unbroken / (double)cnt) : 0.0;
+        if (sed) sed[n - RG.base] = 0.5 * mu_acc;
     }
     const double sum = bbpd::block_sum_fixed<BLOCK>(e_acc, sh);
     if (threadIdx.x == 0) epart[blockIdx.x] = sum;
@@ -2995,7 +3027,7 @@ int region_observables(bbpd_solver *s, int ri, bbpd_observables *o)
 
 namespace {
 
-using DamageFn = void (*)(Params, int, double *, double *);
+using DamageFn = void (*)(Params, int, double *, double *, double *);
 
 template <bool MIXED>
 DamageFn pick_damage(int law)
@@ -3009,27 +3041,35 @@ DamageFn pick_damage(int law)
 }
 
 // damage field (reference node order) and total damage energy of one region
-int region_damage(bbpd_solver *s, int ri, const std::vector<int> &slot_of, double *damage, double *energy)
+// damage field, damage energy and (when asked) the strain energy of one region, in reference order
+int region_damage(bbpd_solver *s, int ri, const std::vector<int> &slot_of, double *damage, double *energy,
+                  double *density = nullptr, double *strain = nullptr)
 {
     BBPD_HIP(hipSetDevice(s->device));
     const Region &RG = s->reg[ri];
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
-    double *dd = nullptr, *ep = nullptr;
+    const bool want_sed = density || strain;
+    double *dd = nullptr, *ep = nullptr, *sd = nullptr;
     BBPD_HIP(hipMalloc((void **)&dd, sizeof(double) * (size_t)RG.nSlots));
     hipError_t e = hipMalloc((void **)&ep, sizeof(double) * (size_t)RG.nblk);
+    if (e == hipSuccess && want_sed) e = hipMalloc((void **)&sd, sizeof(double) * (size_t)RG.nSlots);
     if (e != hipSuccess) {
         (void)hipFree(dd);
+        if (ep) (void)hipFree(ep);
         return bbpd::hip_fail(e, "hipMalloc (damage)");
     }
     DamageFn f = mixed ? pick_damage<true>(s->law) : pick_damage<false>(s->law);
-    hipLaunchKernelGGL(f, dim3(RG.nblk), dim3(BLOCK), 0, s->stream, s->P, ri, dd, ep);
-    std::vector<double> hd(RG.nSlots), he(RG.nblk);
+    hipLaunchKernelGGL(f, dim3(RG.nblk), dim3(BLOCK), 0, s->stream, s->P, ri, dd, ep, sd);
+    std::vector<double> hd(RG.nSlots), he(RG.nblk), hs(want_sed ? RG.nSlots : 0);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(hd.data(), dd, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(he.data(), ep, sizeof(double) * he.size(), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && want_sed)
+        e = hipMemcpyAsync(hs.data(), sd, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(dd);
     (void)hipFree(ep);
+    if (sd) (void)hipFree(sd);
     if (e != hipSuccess) return bbpd::hip_fail(e, "damage kernel");
     if (damage)
         for (size_t n = 0; n < slot_of.size(); n++) damage[n] = hd[slot_of[n] - RG.base];
@@ -3037,6 +3077,17 @@ int region_damage(bbpd_solver *s, int ri, const std::vector<int> &slot_of, doubl
         double t = 0;
         for (double x : he) t = t + x;
         *energy = t;
+    }
+    if (want_sed) {
+        // calculatestrainenergy.m:26: sum(nodalStrainEnergyDensity) * nNodes * cellVolume, the sum in
+        // node order
+        double t = 0;
+        for (size_t n = 0; n < slot_of.size(); n++) {
+            const double v = hs[slot_of[n] - RG.base];
+            if (density) density[n] = v;
+            t = t + v;
+        }
+        if (strain) *strain = t * (double)slot_of.size() * s->P.V;
     }
     return BBPD_OK;
 }
@@ -3049,6 +3100,12 @@ extern "C" int bbpd_solver_damage(bbpd_solver *s, double *damage, double *energy
     BBPD_REQUIRE(!energy || s->law != BBPD_LAW_LINEAR,
                  "damage energy needs a softening law (the brittle latch keeps no stretch history)");
     return region_damage(s, 0, s->slot_of, damage, energy);
+}
+
+extern "C" int bbpd_solver_strain_energy(bbpd_solver *s, double *density, double *energy)
+{
+    BBPD_REQUIRE(s && !s->batch, "bbpd_solver_strain_energy takes a single-problem solver");
+    return region_damage(s, 0, s->slot_of, nullptr, nullptr, density, energy);
 }
 
 extern "C" int bbpd_solver_observables(bbpd_solver *s, bbpd_observables *o)
@@ -4242,6 +4299,12 @@ extern "C" int bbpd_batch_get_bonds(bbpd_solver *s, int region, double *H, doubl
         if (failout) failout[k] = (linear && sgn) ? 0.0 : 1.0;
     }
     return BBPD_OK;
+}
+
+extern "C" int bbpd_batch_strain_energy(bbpd_solver *s, int region, double *density, double *energy)
+{
+    BBPD_REQUIRE(s && s->batch && region >= 0 && region < s->maxRegions && s->active[region], "no such region");
+    return region_damage(s, region, s->rh[region].slot_of, nullptr, nullptr, density, energy);
 }
 
 extern "C" int bbpd_batch_damage(bbpd_solver *s, int region, double *damage, double *energy)

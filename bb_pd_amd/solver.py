@@ -143,6 +143,15 @@ class Solver:
                    "bbpd_solver_damage")
         return d, (e.value if energy else None)
 
+    def strain_energy(self):
+        """(strain energy density per node in the reference order, total strain energy)
+        (calculatestrainenergy.m:10-27)."""
+        d = np.zeros(self.case.nNodes)
+        e = C.c_double()
+        _lib.check(self._L.bbpd_solver_strain_energy(self._h, d.ctypes.data_as(C.c_void_p), C.byref(e)),
+                   "bbpd_solver_strain_energy")
+        return d, e.value
+
     def layout(self) -> str:
         """"stencil" or "records (<reason>)": the bond-table layout chosen at set-up."""
         return self._L.bbpd_solver_layout(self._h).decode()

@@ -284,6 +284,12 @@ int bbpd_solver_set_load_state(bbpd_solver *s, int64_t increment, int64_t itersA
  * branch, where the reference sets H = 1 without the stretch passing sc.  Energy needs a softening
  * law. */
 int bbpd_solver_damage(bbpd_solver *s, double *damage, double *energy);
+/* Strain energy (calculatestrainenergy.m:10-27) of the current state: per node (host, reference
+ * order, may be NULL) the strain energy density 0.5 * sum over its bonds of the micropotential
+ * 0.5 * fail * c * s^2 * L0 * cellVolume * vcf (left to right, BFMULTIPLIER = 1, s from the current
+ * positions in FP64, bonds in BONDLIST order), and in *energy (may be NULL) the reference's total,
+ * sum(density) * nNodes * cellVolume with the sum in node order. */
+int bbpd_solver_strain_energy(bbpd_solver *s, double *nodalStrainEnergyDensity, double *strainEnergy);
 /* device bytes held and the algorithmic HBM bytes one step of the bond kernel moves */
 int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *bond_kernel_bytes, int64_t *n_entries,
                      int64_t *n_padded_entries);
@@ -371,6 +377,7 @@ int bbpd_batch_get_nodal(bbpd_solver *s, int region, double *nodalDisplacement, 
 int bbpd_batch_get_bonds(bbpd_solver *s, int region, double *bondSofteningFactor, double *flagBondSoftening,
                          double *fail);
 int bbpd_batch_damage(bbpd_solver *s, int region, double *damage, double *energy);
+int bbpd_batch_strain_energy(bbpd_solver *s, int region, double *nodalStrainEnergyDensity, double *strainEnergy);
 
 #ifdef __cplusplus
 }

@@ -490,6 +490,28 @@ void or_calculatedamage(const double *BONDLIST, const double *unbroken, const do
     free(cnt);
 }
 
+/* strain energy -- solvers/functions/calculatestrainenergy.m:10-27.  The micropotential of :16 left
+ * to right (BFMULTIPLIER NULL = 1, buildbonddata.m:56-57), added to both nodes of each bond in
+ * BONDLIST order (:19-24), halved (:24), then sum(density) * nNodes * cellVolume (:26). */
+double or_calculatestrainenergy(int nNodes, const double *fail, const double *c, const double *BF, const double *s,
+                                const double *L0, const double *vcf, const double *BONDLIST, int nBonds, double V,
+                                double *density)
+{
+    for (int n = 0; n < nNodes; n++) density[n] = 0;
+    for (int k = 0; k < nBonds; k++) {
+        double mu = 0.5 * fail[k] * (BF ? BF[k] : 1.0) * c[k] * (s[k] * s[k]) * L0[k] * V * vcf[k];
+        int i = (int)BONDLIST[k], j = (int)BONDLIST[nBonds + k];
+        density[i - 1] = density[i - 1] + mu;
+        density[j - 1] = density[j - 1] + mu;
+    }
+    double t = 0;
+    for (int n = 0; n < nNodes; n++) {
+        density[n] = 0.5 * density[n];
+        t = t + density[n];
+    }
+    return t * nNodes * V;
+}
+
 /* dissipated damage energy -- solvers/functions/calculatedamageenergy.m:35-62 */
 void or_calculatedamageenergy(const double *c, const double *s0, const double *sc, const double *L0,
                               const double *s, double k, double alpha, double V, double *de_hist, int nBonds)

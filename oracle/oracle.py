@@ -36,6 +36,8 @@ plausible slip would fail (tests/test_oracle.py):
   smoothstep                     0 at i = 1, a at i = N, zero slope at both ends, symmetric midpoint
   neighbour list / notch         brute-force O(N^2) search; exact and floating-point notch tests
   damage energy, damage field    closed forms at s0 and sc; all-broken node has damage 1
+  strain energy                  hand value on one bond; uniform dilation e of a bulk lattice node
+                                 gives the continuum density 3 E e^2 (c = 12E/(pi delta^4))
 """
 from __future__ import annotations
 
@@ -234,6 +236,16 @@ def calculatedamage(BL, unbroken, nFAM):
     d = np.zeros(nFAM.shape[0])
     lib().or_calculatedamage(_p(BL), _p(unbroken), _p(nFAM), C.c_int(BL.shape[0]), C.c_int(nFAM.shape[0]), _p(d))
     return d
+
+
+def calculatestrainenergy(nNodes, fail, c, s, L0, vcf, BL, V):
+    """(total strain energy, per-node density) of calculatestrainenergy.m:10-27."""
+    d = np.zeros(nNodes)
+    f = lib().or_calculatestrainenergy
+    f.restype = C.c_double
+    e = f(C.c_int(nNodes), _p(_f(fail)), _p(_f(c)), None, _p(_f(s)), _p(_f(L0)), _p(_f(vcf)), _p(_f(BL)),
+          C.c_int(BL.shape[0]), C.c_double(V), _p(d))
+    return e, d
 
 
 def calculatedamageenergy(c, s0, sc, L0, s, k, alpha, V, hist):

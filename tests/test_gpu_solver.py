@@ -686,3 +686,29 @@ def test_damage_field_and_energy_match_oracle(gpu, oracle, law):
                 assert np.array_equal(d, d_ref)
             else:
                 assert np.abs(d - d_ref).max() <= 0.05
+
+
+@pytest.mark.parametrize("law", [_lib.LAW_BILINEAR, _lib.LAW_LINEAR])
+@pytest.mark.parametrize("layout", ["stencil", "records"])
+def test_strain_energy_matches_oracle(gpu, oracle, law, layout):
+    """calculatestrainenergy.m on the device: per-node densities and the total equal the oracle's bit
+    for bit (the stretch of the current positions, bonds added in BONDLIST order, nodes summed in
+    order); the mixed form's state gives the same energy to FP32 accuracy."""
+    case = ec_case(law, steps=200)
+    print(f"case: strain energy, law {law}, {layout}", flush=True)
+    run = oracle.OracleRun(case)
+    with _solver_with_layout(case, _lib.FP64, layout) as sol:
+        sol.step(150)
+        run.step(150)
+        nB = case.nBonds
+        l, dX, dY, dZ, s = (np.zeros(nB) for _ in range(5))
+        oracle.calculatedeformedlength(run.x, case.BONDLIST, case.L0, l, dX, dY, dZ, s)
+        e_ref, d_ref = oracle.calculatestrainenergy(case.nNodes, run.fail, case.c, s, case.L0, case.vcf,
+                                                    case.BONDLIST, case.cellVolume)
+        d, e = sol.strain_energy()
+        assert same(d, d_ref)
+        assert same([e], [e_ref]) and e > 0
+    with _solver_with_layout(case, _lib.MIXED, layout) as sol:
+        sol.step(150)
+        d, e = sol.strain_energy()
+        assert e == pytest.approx(e_ref, rel=2e-3)

@@ -513,3 +513,38 @@ def test_device_order_trajectory_close_to_reference_order(oracle):
     ra, rb = np.array(a.records["reaction_static"]), np.array(b.records["reaction_static"])
     assert np.abs(ra - rb).max() <= 1e-12 * np.abs(ra).max()
     assert np.array(a.records["cn"]) == pytest.approx(np.array(b.records["cn"]), rel=1e-12, abs=1e-300)
+
+
+# ---------------------------------------------------------------- strain energy
+def test_strain_energy_hand_value(oracle):
+    """One bond, s = 0.25, c = 2, L0 = 1, V = 1, vcf = 1: micropotential 0.5*2*0.0625 = 0.0625, each
+    node's density half of it, total sum(density) * nNodes * V = 0.0625 * 2 = 0.125."""
+    BL = np.asfortranarray([[1.0, 2.0]])
+    e, d = oracle.calculatestrainenergy(2, np.ones(1), np.array([2.0]), np.array([0.25]), np.ones(1), np.ones(1),
+                                        BL, 1.0)
+    assert list(d) == [0.03125, 0.03125]
+    assert e == 0.125
+
+
+def test_strain_energy_dilation_bulk_node(oracle):
+    """Uniform dilation s = e of every bond with c = c0 and vcf = 1: a bulk node's density is
+    1/2 * sum over its bonds of 1/2 c0 e^2 |xi| V, with the sum of |xi| over the integer lattice
+    offsets inside the horizon counted here by brute force (DX = 1); doubling e quadruples every
+    density exactly."""
+    from bb_pd_amd import beam
+    case = beam.build_beam("t", 15, 15, 15, 1.0, law=1, sc=1.0, integrator=0, schedule=0)
+    e0 = 1e-3
+    c = np.full(case.nBonds, case.c0)
+    one = np.ones(case.nBonds)
+    _, d = oracle.calculatestrainenergy(case.nNodes, case.fail, c, np.full(case.nBonds, e0), case.L0, one,
+                                        case.BONDLIST, case.cellVolume)
+    r = np.arange(-4, 5)
+    o = np.sqrt((r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2).astype(float)).ravel()
+    o = o[(o > 0) & (o <= case.horizon)]
+    assert o.size == 122
+    centre = 7 + 15 * (7 + 15 * 7)
+    expect = 0.5 * 0.5 * case.c0 * e0 ** 2 * math.fsum(o) * case.cellVolume
+    assert d[centre] == pytest.approx(expect, rel=1e-13)
+    _, d2 = oracle.calculatestrainenergy(case.nNodes, case.fail, c, np.full(case.nBonds, 2 * e0), case.L0, one,
+                                         case.BONDLIST, case.cellVolume)
+    assert np.array_equal(d2, 4 * d)
