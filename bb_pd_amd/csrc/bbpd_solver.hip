@@ -93,6 +93,7 @@ struct Region {
     double s0, s1, sc;             // scalar law parameters
     double s0lo, sclo;             // stencil layout: the smallest streamed s0 / sc of the region
     int body0, nBodies;            // contact bodies body0 .. body0 + nBodies - 1
+    int seq_contact;               // the families overlap: one step-end block applies them in order
     int schedule, nTimeSteps, ipi; // load schedule (ipi: iterations per increment, at most, for the
     double applied, increment;     // equilibrium staircase)
     double eq_tol;                 // equilibrium staircase: advance below this equilibrium ratio ...
@@ -1194,7 +1195,9 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
     const double *XX[3] = {P.X0, P.X1, P.X2};
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double total = 0, rs = 0;
-    if (bi >= 0) {
+    // one body's contact (calculatecontactforce.m:62-121): its reaction in `total`, and for body 0
+    // the static reaction in `rs`
+    auto body = [&](const int bi) {
         const Body &B = P.bodies[RG.body0 + bi];
         const double disp = schedule_disp(RG, RS, it);
         const double cx = B.cx, czm = B.cz + (B.moves ? disp : 0.0), R = B.R;
@@ -1298,12 +1301,26 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
         }
         // the static reaction: the threads' shares summed in a fixed tree
         if (bi == 0) rs = bbpd::block_sum_fixed<1024>(part, vals);
+    };
+    if (bi >= 0) {
+        body(bi);
+    } else if (bi == -2) {
+        // overlapping families: this block applies the bodies in order, as the reference's calls
+        // follow one another (dynamicsolverdisplacementcontrolled.m:108-112); a member shared with an
+        // earlier body is seen as that body left it (the barrier orders the block's global writes)
+        for (int b = 0; b < RG.nBodies; b++) {
+            total = 0;
+            body(b);
+            if (threadIdx.x == 0) RS.reaction[b] = total;
+            if (b == 0 && threadIdx.x == 0) RS.reaction_static = rs;
+            __syncthreads();
+        }
     }
     if (threadIdx.x == 0) {
         if (bi >= 0) RS.reaction[bi] = total;
         if (bi == 0) RS.reaction_static = rs;
         __threadfence();
-        const unsigned need = (unsigned)(RG.nBodies > 0 ? RG.nBodies : 1);
+        const unsigned need = (unsigned)(RG.nBodies > 0 && !RG.seq_contact ? RG.nBodies : 1);
         const unsigned ticket = atomicAdd(&RS.done, 1u);
         last_block = (ticket == need - 1);
     }
@@ -2813,9 +2830,12 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.hist_cap = HIST_CAP;
     R0.stop_frac = 0;
     R0.stop_disp = 0;
-    // the bodies are projected concurrently (one block each), so a node may belong to one family
-    // only; the reference applies them in sequence, which is the same for disjoint families
+    // disjoint families are projected concurrently (one step-end block each); when families share
+    // members, one block applies the bodies in order, as the reference's calls follow one another.
+    // A family lists each member once (its members are projected in parallel).
     std::vector<unsigned char> in_family(nN, 0);
+    std::vector<int> owner(nN, -1);
+    R0.seq_contact = 0;
     std::vector<Body> hb(std::max(pr->nBodies, 1));
     for (int bdy = 0; bdy < pr->nBodies; bdy++) {
         int nf = pr->nFamily[bdy];
@@ -2828,11 +2848,13 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
                 bbpd::set_error("contact family holds an id outside 1..nNodes");
                 return fail(BBPD_ERR_ARG);
             }
-            if (in_family[id - 1]) {
-                bbpd::set_error("contact families overlap (node " + std::to_string(id) +
-                                "): the fused solver needs disjoint families");
-                return fail(BBPD_ERR_UNSUPPORTED);
+            if (owner[id - 1] == bdy) {
+                bbpd::set_error("contact family " + std::to_string(bdy) + " lists node " + std::to_string(id) +
+                                " twice");
+                return fail(BBPD_ERR_ARG);
             }
+            if (owner[id - 1] >= 0) R0.seq_contact = 1;
+            owner[id - 1] = bdy;
             in_family[id - 1] = 1;
             fs[t] = s->slot_of[id - 1];
         }
@@ -2862,7 +2884,10 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         std::vector<int> blkreg(nblk, 0), grid(nblk);
         std::iota(grid.begin(), grid.end(), 0);
         std::vector<int2> se;
-        for (int bdy = 0; bdy < std::max(pr->nBodies, 1); bdy++) se.push_back(make_int2(0, pr->nBodies > 0 ? bdy : -1));
+        if (R0.seq_contact) se.push_back(make_int2(0, -2));
+        else
+            for (int bdy = 0; bdy < std::max(pr->nBodies, 1); bdy++)
+                se.push_back(make_int2(0, pr->nBodies > 0 ? bdy : -1));
         int act = 0;
         if ((rc = dalloc(&s->d_reg, 1, s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_rst, 1, s->owned))) return fail(rc);
@@ -3661,7 +3686,9 @@ int batch_rebuild(bbpd_solver *s)
             grid.push_back(RG.blk0 + b);
         }
         if (RG.nBodies == 0) se.push_back(make_int2(r, -1));
-        for (int b = 0; b < RG.nBodies; b++) se.push_back(make_int2(r, b));
+        else if (RG.seq_contact) se.push_back(make_int2(r, -2));
+        else
+            for (int b = 0; b < RG.nBodies; b++) se.push_back(make_int2(r, b));
         act.push_back(r);
         maxtab = std::max(maxtab, (size_t)RG.ntab);
     }
@@ -4014,6 +4041,8 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     std::vector<Body> hb(BBPD_MAX_BODIES);
     std::vector<int> fam;
     std::vector<unsigned char> in_family(nN, 0);
+    std::vector<int> owner(nN, -1);
+    int seq_contact = 0; // families that share members are applied in order by one block
     int *fambase = s->d_fampool + (size_t)rid * RB_FAM;
     for (int bdy = 0; bdy < pr->nBodies; bdy++) {
         const int nf = pr->nFamily[bdy];
@@ -4028,10 +4057,12 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
                 bbpd::set_error("contact family holds an id outside 1..nNodes");
                 return BBPD_ERR_ARG;
             }
-            if (in_family[id - 1]) {
-                bbpd::set_error("contact families overlap: the fused solver needs disjoint families");
-                return BBPD_ERR_UNSUPPORTED;
+            if (owner[id - 1] == bdy) {
+                bbpd::set_error("a contact family lists a node twice");
+                return BBPD_ERR_ARG;
             }
+            if (owner[id - 1] >= 0) seq_contact = 1;
+            owner[id - 1] = bdy;
             in_family[id - 1] = 1;
             fam.push_back(base + sp.slot[id - 1]);
         }
@@ -4087,6 +4118,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.sclo = pr->sc && nB > 0 ? *std::min_element(pr->sc, pr->sc + nB) : 0.0;
     RG.body0 = rid * BBPD_MAX_BODIES;
     RG.nBodies = pr->nBodies;
+    RG.seq_contact = seq_contact;
     RG.schedule = pr->schedule;
     RG.nTimeSteps = pr->nTimeSteps;
     RG.ipi = pr->itersPerIncrement > 0 ? pr->itersPerIncrement : 1;

@@ -95,8 +95,8 @@ def test_beam_geometry():
 
 
 def test_smoke_case_has_disjoint_contact_families():
-    """The fused solver rejects overlapping contact families (a node projected by two bodies in one
-    step); the smoke run's beam must not have any."""
+    """The smoke run's beam has disjoint contact families, so its bodies' step-end blocks run
+    concurrently (overlapping families take the sequential path, tested on the GPU)."""
     import __graft_entry__ as G
     case = G.smoke_case()
     fams = [set(np.asarray(b.family).astype(int).tolist()) for b in case.bodies]

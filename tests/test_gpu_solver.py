@@ -712,3 +712,55 @@ def test_strain_energy_matches_oracle(gpu, oracle, law, layout):
         sol.step(150)
         d, e = sol.strain_energy()
         assert e == pytest.approx(e_ref, rel=2e-3)
+
+
+def overlapping_case(integrator):
+    """A beam whose contact families share members: a second penetrator with the first one's family and
+    a 5% larger radius.  A member inside both circles is projected by the first, then pushed out by the
+    second, as the reference's contact calls follow one another (dynamicsolverdisplacementcontrolled.m:
+    108-112); in the other order the result differs."""
+    import copy
+    case = ec_case(_lib.LAW_BILINEAR, steps=200) if integrator == _lib.EULER_CROMER else adr_case(200)
+    b2 = copy.deepcopy(case.bodies[0])
+    b2.R = b2.R * 1.05
+    case.bodies = case.bodies + [b2]
+    return case
+
+
+@pytest.mark.parametrize("integrator", [_lib.EULER_CROMER, _lib.ADR])
+def test_overlapping_contact_families_in_order(gpu, oracle, integrator):
+    case = overlapping_case(integrator)
+    print(f"case: overlapping contact families, integrator {integrator}", flush=True)
+    run = oracle.OracleRun(case, device_order=True)
+    with Solver(case, _lib.FP64) as sol:
+        sol.step(150)
+        run.step(150)
+        compare_state(sol, run, exact=True)
+        h = sol.history()
+        assert same(h["reaction"], run.reactions)
+        assert same(h["reaction_static"], run.records["reaction_static"])
+        # the second penetrator moved members the first had placed
+        fam = case.bodies[0].family.astype(int) - 1
+        d = np.hypot(run.x[fam, 0] - case.bodies[3].cx,
+                     run.x[fam, 2] - (case.bodies[3].cz + run.records["disp"][-1]))
+        assert np.isclose(d, case.bodies[3].R, rtol=1e-12).any()
+
+
+def test_overlapping_contact_families_in_a_batch(gpu):
+    from bb_pd_amd.batch import Batch
+    case = overlapping_case(_lib.ADR)
+    other = adr_case(200)
+    with Batch(case, _lib.FP64, max_regions=2, max_slots=8192) as b:
+        r0 = b.admit(other)
+        r1 = b.admit(case)
+        b.step(120)
+        got = b.nodal(r1)
+        h = b.history(r1)
+    with Solver(case, _lib.FP64) as s:
+        assert s.layout() == "stencil"
+        s.step(120)
+        for x, y in zip(s.nodal(), got):
+            assert same(x, y)
+        hs = s.history()
+        for k in ("reaction", "reaction_static", "deflection", "cmod"):
+            assert same(hs[k], h[k]), k
