@@ -136,8 +136,8 @@ struct __align__(32) StOff {
     union {
         struct {
             float x, y, z, L0; // mixed form: xi = h * d rounded to FP32, and |xi|
-            float vcf;         // mixed form: volume correction factor of |h * d|
-            float L0sq;        // mixed form: |xi|^2
+            float vr;          // mixed form: vcf / |xi|, vcf the volume correction factor of |h * d|
+            float rq;          // mixed form: 1 / (2 |xi|^2)
         };
         struct {
             double Ln, hn, rn; // FP64 form: a nominal length L of the offset, RN(0.5 / L), RN(1 / L)
@@ -601,6 +601,7 @@ struct StEnt {
     T p0, p1; // streamed s0, sc
     StOff st; // per-offset data (the same for every lane)
     unsigned long long mk; // the wavefront's lanes that have the bond
+    bool hb;  // the slice's history bit of the offset (clear: every entry holds +0, not loaded)
 };
 
 // one stencil row of neighbour records in flight: sites 0 .. 63 (a) and 64 .. 63 + 2 rx (b)
@@ -744,6 +745,11 @@ __device__ __forceinline__ bool law_active(T s, T thr, bool flag)
     return s > thr;
 }
 
+// Mixed form, stencil layout: the stretch by its Taylor series in x = (|y|^2 - |xi|^2) / (2 |xi|^2)
+// for |x| at most this, else by a reciprocal square root and a reciprocal.  Both round to within a
+// few ulps of the exact stretch.
+constexpr float MIX_SERIES_X = 0x1p-9f;
+
 // Launch variants, picked per solver by an autotuner: OCC is the minimum waves per SIMD the compiler
 // must allow (1 = its own choice), which bounds the registers per wave, trading instruction-level
 // for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.  PV says
@@ -857,16 +863,24 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(w >> 32));
         return ((unsigned long long)hi << 32) | lo;
     };
-    const unsigned long long hz0 = uniform64(hzp[0]), hz1 = uniform64(hzp[1]);
-    auto hbit = [&](int o) { return o < 128 && (((o < 64 ? hz0 : hz1) >> (o & 63)) & 1ull) != 0; };
-    // offset o's loads; o runs to noff: the entry arrays end at least two slices' worth past the
-    // last entry and the per-offset table ST_PAD records past the last offset
-    auto load = [&](StEnt<T> &d, int o) {
-        const uint32_t so = (uint32_t)o * (SLICE * TW);
-        if (hbit(o)) d.h = bload_nt<T>(rh, lb, so);
+    // the two words as a 128-bit shift register, shifted right by 4 per group of four offsets: bit k
+    // is then the bit of the group's offset k (k = 4: the next group's first)
+    unsigned long long hlo = uniform64(hzp[0]), hhi = uniform64(hzp[1]);
+    // The offsets in groups of four: the group's per-offset records, lane masks and history bytes
+    // are at constant offsets from three bases that advance once per group (sp, mp, hso), so a load
+    // needs no scalar address arithmetic of its own.
+    const StOff *sp = sst;
+    const unsigned long long *mp = mks;
+    uint32_t hso = 0; // byte offset of the group's first offset in the slice's history
+    // the loads of the group's offset k (0 .. 4); the offsets run to noff + 1: the entry arrays end at
+    // least two slices' worth past the last entry and the per-offset tables ST_PAD records past the
+    // last offset
+    auto load = [&](StEnt<T> &d, int k, uint32_t bits) {
+        d.hb = ((bits >> k) & 1u) != 0u;
+        if (d.hb) d.h = bload_nt<T>(rh, lb + (uint32_t)k * (SLICE * TW), hso);
         else d.h = (T)0;
-        d.st = sload(sst, o);
-        d.mk = sload(mks, o);
+        d.st = sload(sp, k);
+        d.mk = sload(mp, k);
     };
     // One entry.  The FP64 form adds its force to the row sum here; the mixed form adds it to the
     // FP32 sum `a` of a group of up to four entries, which the loop adds to the FP64 row sum.
@@ -876,15 +890,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     // two registers they hold across the row cost more in spills than the row saves.
     int cyz = INT_MIN;
     double eyy = 0, ezz = 0;
-    auto compute = [&](const StEnt<T> &d, int o, bool bit, float3 &a) {
+    auto compute = [&](const StEnt<T> &d, int o, int k, bool bit, float3 &a) {
         if (!MIXED && PV != PV_GENERAL && d.st.dyz != cyz) {
             cyz = d.st.dyz;
             const double ey = Xn1 - P.hy * (double)(ky + d.st.dy()), ez = Xn2 - P.hz * (double)(kz + d.st.dz());
             eyy = ey * ey;
             ezz = ez * ez;
         }
+        // the record's last word (unused by the mixed form) kept live where the record is used: the
+        // record then arrives by one 32-byte scalar load rather than by three narrower ones
+        if constexpr (MIXED) __asm__ volatile("" ::"s"(d.st.dyz));
         if (!bit) return;
-        const uint32_t so = (uint32_t)o * (SLICE * TW);
+        // the entry's byte offset in the slice's history (and in the parameter streams)
+        const uint32_t so = hso + (uint32_t)k * (SLICE * TW);
         const double *nb = &s_row[wv][0] + (lane + rx + d.st.dx);
         const double nbx = nb[0], nby = nb[ST_ROW], nbz = nb[2 * ST_ROW];
         const int key = krow + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
@@ -931,7 +949,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
                 __builtin_bit_cast(typename std::conditional<MIXED, uint32_t, unsigned long long>::type, d.h)) {
                 bstore<T>(hv, rh, lb, so);
                 // the first history word other than +0 at this offset of the slice
-                if (!hbit(o)) atomicOr(hzp + (o >> 6), 1ull << (o & 63));
+                if (!d.hb) atomicOr(hzp + (o >> 6), 1ull << (o & 63));
             }
         };
         if constexpr (!MIXED) {
@@ -999,22 +1017,34 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             f1 += gy;
             f2 += gz;
         } else {
-            const float kap = c * d.st.vcf;
             const float et0 = (float)(nbx - qn0), et1 = (float)(nby - qn1), et2 = (float)(nbz - qn2);
             const float y0 = d.st.x + et0, y1 = d.st.y + et1, y2 = d.st.z + et2;
-            // |y|^2 carries 1e-30 (below every representable bond's square by many orders, so it
-            // changes no realistic length): 1/l stays finite for a bond of zero length, whose force
-            // p * y is then 0, the reference's NaN -> 0 (calculatebondforcesMex.c:38)
-            const float l2 = fmaf(y0, y0, fmaf(y1, y1, fmaf(y2, y2, 1e-30f)));
-            const float rl = __builtin_amdgcn_rsqf(l2);
-            const float l = l2 * rl;
-            // |y|^2 - |xi|^2 = eta . (2 xi + eta), and |xi| (l + |xi|) = |xi| l + |xi|^2
+            // |y|^2 - |xi|^2 = eta . (2 xi + eta), without cancellation; with x = that / (2 |xi|^2),
+            // |y| = |xi| sqrt(1 + 2x), the stretch is s = sqrt(1 + 2x) - 1 and g = s / (1 + s) =
+            // s |xi| / |y| carries the force's 1 / |y|
             const float num = fmaf(et0, fmaf(2.0f, d.st.x, et0),
                                    fmaf(et1, fmaf(2.0f, d.st.y, et1), et2 * fmaf(2.0f, d.st.z, et2)));
-            const float s = num * __builtin_amdgcn_rcpf(fmaf(d.st.L0, l, d.st.L0sq));
+            const float x = num * d.st.rq;
+            // |x| <= 2^-9: Taylor series, s = x - x^2/2 + x^3/2 (the next term, -5 x^4 / 8, is below
+            // 2^-27 |s|) and g = s - s^2 + s^3 (next s^4, below 2^-26 |g|).  The choice is the
+            // bond's own (x is the same from either end), so both ends store the same history.
+            float s = x * fmaf(x, fmaf(x, 0.5f, -0.5f), 1.0f);
+            float g = fmaf(s, fmaf(s, s, -s), s);
+            if (!(fabsf(x) <= MIX_SERIES_X)) {
+                // |y|^2 carries 1e-30 (below every representable bond's square by many orders, so it
+                // changes no realistic length): 1/l stays finite for a bond of zero length, whose
+                // force p * y is then 0, the reference's NaN -> 0 (calculatebondforcesMex.c:38)
+                const float l2 = fmaf(y0, y0, fmaf(y1, y1, fmaf(y2, y2, 1e-30f)));
+                const float rl = __builtin_amdgcn_rsqf(l2);
+                const float l = l2 * rl;
+                s = num * __builtin_amdgcn_rcpf(d.st.L0 * (l + d.st.L0));
+                g = (s * rl) * d.st.L0;
+            }
             law(s);
-            // kap (1 - H) s / l; a failed bond of the brittle law carries no force
-            const float p = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0f : fmaf(-kap, fabsf(hv), kap) * (s * rl);
+            // kap (1 - H) s / l = (c vcf / |xi|) (1 - H) g; a failed bond of the brittle law carries
+            // no force
+            const float kap = c * d.st.vr;
+            const float p = (LAW == BBPD_LAW_LINEAR && flag) ? 0.0f : fmaf(-kap, fabsf(hv), kap) * g;
             a.x = fmaf(p, y0, a.x);
             a.y = fmaf(p, y1, a.y);
             a.z = fmaf(p, y2, a.z);
@@ -1040,28 +1070,34 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             }
         };
         StEnt<T> A, B;
-        load(A, 0);
+        load(A, 0, (uint32_t)hlo);
         // the lane masks are wavefront-uniform: a lane's bit is tested by running the entry under
         // the mask as the execution mask
-        // offsets o, o + 1 with loads one offset ahead
-        auto pair = [&](int o, float3 &a) {
-            load(B, o + 1);
+        // the group's offsets k, k + 1 (o, o + 1) with loads one offset ahead
+        auto pair = [&](int o, int k, uint32_t bits, float3 &a) {
+            load(B, k + 1, bits);
             next_row(o);
-            compute(A, o, __builtin_amdgcn_inverse_ballot_w64(A.mk), a);
-            load(A, o + 2);
+            compute(A, o, k, __builtin_amdgcn_inverse_ballot_w64(A.mk), a);
+            load(A, k + 2, bits);
             next_row(o + 1);
-            compute(B, o + 1, __builtin_amdgcn_inverse_ballot_w64(B.mk), a);
+            compute(B, o + 1, k + 1, __builtin_amdgcn_inverse_ballot_w64(B.mk), a);
         };
         // groups of four offsets: the mixed form sums a group in FP32 and adds that to the FP64 row sum
         for (int o = 0; o < noff; o += 4) {
+            const uint32_t bits = (uint32_t)hlo;
             float3 a = make_float3(0.0f, 0.0f, 0.0f);
-            pair(o, a);
-            if (o + 2 < noff) pair(o + 2, a);
+            pair(o, 0, bits, a);
+            if (o + 2 < noff) pair(o + 2, 2, bits, a);
             if constexpr (MIXED) {
                 f0 += (double)a.x;
                 f1 += (double)a.y;
                 f2 += (double)a.z;
             }
+            sp += 4;
+            mp += 4;
+            hso += 4 * SLICE * TW;
+            hlo = (hlo >> 4) | (hhi << 60);
+            hhi >>= 4;
         }
     }
     // the block reduction of finish_node (ADR) reuses the row buffers once every wavefront is past
@@ -2101,9 +2137,12 @@ std::vector<StOff> offset_records(const std::vector<int4> &off, const double *h,
             if (L0 <= hmr) vcf = 1;
             else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
             else vcf = 0;
-            const float L0f = std::sqrt(std::fma(x0, x0, std::fma(x1, x1, x2 * x2)));
-            R.x = x0, R.y = x1, R.z = x2, R.L0 = L0f, R.vcf = (float)vcf;
-            R.L0sq = (float)((double)L0f * (double)L0f);
+            // |xi|^2 of the FP32 vector, in FP64
+            const double q = ((double)x0 * x0 + (double)x1 * x1) + (double)x2 * x2;
+            const float L0f = (float)std::sqrt(q);
+            R.x = x0, R.y = x1, R.z = x2, R.L0 = L0f;
+            R.vr = q > 0 ? (float)(vcf / std::sqrt(q)) : 0.0f;
+            R.rq = q > 0 ? (float)(0.5 / q) : 0.0f;
             continue;
         }
         const std::vector<double> ex = diffs(0, off[o].y), ey = diffs(1, off[o].z), ez = diffs(2, off[o].w);
