@@ -646,6 +646,8 @@ def main():
     ap.add_argument("--max-study-steps", type=int, default=120000)
     ap.add_argument("--launch-variant", type=int, default=None, help="counter runs: fixed bond-kernel variant")
     ap.add_argument("--state-from", default=None, help="counter runs: restore this state file first")
+    ap.add_argument("--save-state", default=None,
+                    help="single-form runs: write the state after the timed steps to this .npz (for counter runs)")
     ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-state", action="append", default=[], help=argparse.SUPPRESS)
     ap.add_argument("--batch", action="store_true",
@@ -730,6 +732,8 @@ def main():
     if args.form != "both":
         run = Run(case, _lib.FP64 if args.form == "fp64" else _lib.MIXED)
         res = run.timed(args.steps, args.warmup, args.chunk)
+        if args.save_state:
+            save_state(args.save_state, run.sol.checkpoint())
         run.close()
         print(json.dumps({"form": args.form, "result": res}), flush=True)
         return
