@@ -591,11 +591,12 @@ def run_serve(n_requests: int, hbm_gb: float, ipi: int, increment_um: float, loa
     out = {}
     for name, prec in (("mixed", _lib.MIXED), ("fp64", _lib.FP64)):
         reqs = [Request(i, params=p) for i, p in enumerate(params)]
-        cap = serve(reqs, prec, hbm_gb * 1e9, chunk=ipi, ipi=ipi, increment_um=increment_um, closed_loop=True)
+        cap = serve(reqs, prec, hbm_gb * 1e9, chunk=ipi, ipi=ipi, increment_um=increment_um, closed_loop=True,
+                    log=log)
         lam = load * cap["requests_per_s"]
         arr = np.cumsum(np.random.default_rng(7).exponential(1 / lam, n_requests))
         reqs = [Request(i, params=p, arrival=float(a)) for i, (p, a) in enumerate(zip(params, arr))]
-        res = serve(reqs, prec, hbm_gb * 1e9, chunk=ipi, ipi=ipi, increment_um=increment_um)
+        res = serve(reqs, prec, hbm_gb * 1e9, chunk=ipi, ipi=ipi, increment_um=increment_um, log=log)
         log(f"{name}: capacity {cap['requests_per_s']:.2f} req/s; at {load:.0%}: p50 {res['p50_s']:.2f} s, "
             f"p99 {res['p99_s']:.2f} s, {res['bond_updates_per_s']:.3e} bond updates/s, {res['slots']} slots")
         steps = [r.steps for r in res["done"]]

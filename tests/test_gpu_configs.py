@@ -194,3 +194,46 @@ def test_size_effect_batch_regions(gpu, oracle):
     assert same(bonds[0][0], run.H) and np.array_equal(bonds[0][1], run.flag)
     for k in ("reaction_static", "cn", "disp"):
         assert same(hist[0][k], run.records[k]), k
+
+
+def _free_run_loads(case, precision, chunk=2000, max_steps=120_000, stop=None):
+    """Converged loads (static reaction at the load-increment ends) of a free run in graph chunks,
+    until `stop(loads)` or max_steps."""
+    loads = []
+    with Solver(case, precision) as sol:
+        step = 0
+        while step < max_steps and not (stop and stop(loads)):
+            sol.run_graph(chunk, 1)
+            sol.sync()
+            step += chunk
+            h = sol.history()
+            loads += list(h["reaction_static"][h["inc_end"]])
+        return np.array(loads), step
+
+
+def test_mixed_peak_load_within_half_percent(gpu):
+    """The FP32-mixed form free-running against FP64 on the dx = 2.5 mm notched trilinear beam
+    (8.9 M bonds) under the equilibrium staircase (5 um per increment, equilibrium ratio 1e-4, 50 to
+    6000 iterations): the peak of the converged load curve agrees within 0.5%, and the curve up to the
+    peak within 1% of the peak (the mixed form's stated gate)."""
+    case = beam.case_trilinear_dx2p5(schedule=2, iters_per_increment=6000, increment=-5e-6, eq_tol=1e-4,
+                                     iters_per_increment_min=50)
+    print(f"case: dx2.5 trilinear, equilibrium staircase, {case.nBonds} bonds", flush=True)
+
+    def past_peak(loads):
+        if len(loads) < 3:
+            return False
+        m = int(np.argmax(loads))
+        return len(loads) - 1 - m >= 2 and loads[-1] < 0.9 * loads[m]
+
+    l64, n64 = _free_run_loads(case, _lib.FP64, stop=past_peak)
+    assert past_peak(l64), f"FP64 run did not pass its peak in {n64} steps: {l64}"
+    lm, nm = _free_run_loads(case, _lib.MIXED, stop=lambda l: len(l) >= len(l64))
+    n = min(len(l64), len(lm))
+    p64, pm = int(np.argmax(l64[:n])), int(np.argmax(lm[:n]))
+    err = (lm[pm] - l64[p64]) / l64[p64]
+    curve = np.abs(lm[:p64 + 1] - l64[:p64 + 1]).max() / l64[p64]
+    print(f"  FP64 peak {l64[p64]:.2f} N at increment {p64 + 1} ({n64} steps); mixed {lm[pm]:.2f} N at "
+          f"{pm + 1}: peak error {err:+.2e}, curve to the peak {curve:.2e}", flush=True)
+    assert abs(err) <= 5e-3
+    assert curve <= 1e-2
