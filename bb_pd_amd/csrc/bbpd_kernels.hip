@@ -579,24 +579,34 @@ constexpr float MIX_SERIES_X = 0x1p-9f;
 // for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.  PV says
 // where s0 and sc come from: PV_SCALAR the region's scalars, PV_TABLE both from key tables,
 // PV_GENERAL each from its scalar, key table or per-entry stream as the solver's modes say.
-template <bool MIXED, int LAW, int OCC, bool REMAP, int PV>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) void k_bond_st(Params P)
+template <bool MIXED, int LAW, int OCC, bool REMAP, int PV, bool TILE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && OCC > 1 ? 8 : OCC))) void k_bond_st(Params P)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
     constexpr uint32_t TW = sizeof(T);
-    // per wavefront, a row buffer: x, y, z and the key word (low word) of ST_ROW sites.  One buffer
-    // suffices: a wavefront's LDS accesses complete in program order, so staging the next row
+    // per wavefront, a row buffer: x, y, z and the key word (low word) of the staged sites, RS sites
+    // per component: one row of ST_ROW sites (slices), or TILE_L lines of TILE_W sites (tiles).  One
+    // buffer suffices: a wavefront's LDS accesses complete in program order, so staging the next row
     // (after the last read of this one) cannot overtake a read.
-    __shared__ double s_row[BLOCK / SLICE][4 * ST_ROW];
+    constexpr int RS = TILE ? TILE_W : ST_ROW;
+    __shared__ double s_row[BLOCK / SLICE][TILE ? TILE_L * 4 * TILE_W : 4 * ST_ROW];
     extern __shared__ double tabs[]; // key tables (FP64, or FP32 in the mixed form)
     const int gb = launch_block(P, REMAP);
     if (gb < 0) return;
-    const int n = gb * BLOCK + threadIdx.x;
     const Region &RG = P.reg[P.blkreg[gb]];
+    const int lane = (int)(threadIdx.x % SLICE);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / SLICE));
+    // tiles: wavefront wv of the region's block takes tile (block * 4 + wv); a wavefront past the last
+    // tile has no node (it still joins the block's reductions)
+    const int tile = TILE ? (gb - RG.blk0) * (BLOCK / SLICE) + wv : 0;
+    const bool wave_on = !TILE || tile < RG.ntile;
+    const int t0 = TILE && wave_on ? RG.base + tile_slot(tile, 0, RG.tnx) : 0; // the tile's first slot
+    const int n = TILE ? (wave_on ? t0 + lane % TILE_X + RG.tnx * (lane / TILE_X) : RG.base)
+                       : gb * BLOCK + (int)threadIdx.x;
     const int base = RG.base, noff = RG.noff, off0 = RG.off0, kg = RG.kg, tab0 = RG.tab0, ntab = RG.ntab;
     const int ts0 = RG.ts0, tsc = RG.tsc;
     const bool fast64 = !MIXED && RG.fast64;
-    const bool real = n - base < RG.nN;
+    const bool real = TILE ? wave_on : n - base < RG.nN;
     const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = law_threshold<LAW, T>(rs0, rsc, rs1);
     const T rs0lo = (T)RG.s0lo, rsclo = (T)RG.sclo;
     const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
@@ -633,14 +643,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     const int krow = gme * kg * KSCALE + tabl; // the node's row of the key table (index or LDS address)
     // X == h * k exactly (checked at set-up)
     const double Xn0 = P.hx * (double)kx, Xn1 = P.hy * (double)ky, Xn2 = P.hz * (double)kz;
-    // this wavefront's slice of the entry arrays: entry (o, lane) at byte (o * SLICE + lane) * TW
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / SLICE));
-    const long long eslice = RG.e0 + ((long long)((gb * BLOCK - base) / SLICE) + wv) * noff * SLICE;
-    const uint32_t sbytes = (uint32_t)noff * SLICE * TW;
+    // the entry arrays: entry (slice, o, lane) of the region at ((slice * noff + o) * SLICE + lane) * TW.
+    // Slices: the wavefront's slice, lane offset lane * TW.  Tiles: the region's entries, the lane's
+    // own slice and lane in the offset (set-up keeps them below 2^31 bytes).
+    const long long eslice = TILE ? RG.e0 : RG.e0 + ((long long)((gb * BLOCK - base) / SLICE) + wv) * noff * SLICE;
+    const uint32_t sbytes = TILE ? (uint32_t)(((long long)RG.nSlots * noff + 2 * SLICE) * TW) : (uint32_t)noff * SLICE * TW;
     const brsrc rh = make_rsrc((const T *)P.hs + eslice, sbytes);
     const brsrc r0 = PV == PV_GENERAL && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
     const brsrc r1 = PV == PV_GENERAL && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
-    const uint32_t lb = (threadIdx.x % SLICE) * TW;
+    const uint32_t lb = TILE ? (uint32_t)(((n - base) / SLICE * noff * SLICE + (n - base) % SLICE) * (int)TW)
+                             : (uint32_t)lane * TW;
     // Neighbour records, staged per stencil row (the offsets that share dy and dz, consecutive in
     // offset order).  For the slice starting at slot n0, row r covers the slots
     // n0 + delta_r - rx .. n0 + delta_r + 63 + rx; the wavefront loads them once into its LDS row
@@ -648,21 +660,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     // loads are in flight while a row is computed.  Slot offsets are 32-bit: a site before slot 0
     // wraps to an offset past every valid one (set-up keeps slots below 2^26), and a read outside
     // the slot array returns zeros (such a site is never a bond).
+    // Tiles stage each stencil row as TILE_L lines of TILE_W sites, lines at the slot stride tnx: line j
+    // of row r holds the sites t0 + delta_r + j * tnx - TILE_RX + i, and lane (x, y) of the tile reads
+    // line y, site x + TILE_RX + dx.  The lines sit in LDS line slots (sh + j) & 3: when row r + 1 is
+    // row r moved by one line (delta_{r+1} - delta_r == tnx, the next dy of the same dz), it shares
+    // three of its lines, so only its last line is loaded and sh advances by one; otherwise all four
+    // lines are loaded.
     const brsrc rp = make_rsrc(pos, (uint32_t)P.nSlots * (uint32_t)sizeof(NodeRec));
-    const int lane = (int)(threadIdx.x % SLICE);
     const int n0 = gb * BLOCK + wv * SLICE;
-    const int rx = RG.rx, nrow = RG.nrow;
+    const int rx = RG.rx, nrow = RG.nrow, tnx = RG.tnx;
     const int *rows = reinterpret_cast<const int *>(P.srow + RG.row0); // (count, delta) per row
     auto fetch = [&](RowRecs &R, int dl) {
         const uint32_t sb = (uint32_t)(n0 + dl - rx);
         R.a = bload_rec(rp, (sb + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
         if (lane < 2 * rx) R.b = bload_rec(rp, (sb + SLICE + (uint32_t)lane) * (uint32_t)sizeof(NodeRec));
     };
-    auto put = [&](double *w, int site, const NodeRec &q) {
+    auto put = [&](double *w, int site, const NodeRec q) {
         w[site] = q.x;
-        w[ST_ROW + site] = q.y;
-        w[2 * ST_ROW + site] = q.z;
-        *reinterpret_cast<int *>(w + 3 * ST_ROW + site) = (int)(q.q >> G_SHIFT) * KSCALE;
+        w[RS + site] = q.y;
+        w[2 * RS + site] = q.z;
+        *reinterpret_cast<int *>(w + 3 * RS + site) = (int)(q.q >> G_SHIFT) * KSCALE;
         // the record's low lattice word is not used; keeping it live until here stops the register
         // allocator from reusing its register while the load is in flight (which would make the
         // wavefront wait for every outstanding load)
@@ -673,15 +690,56 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         put(w, lane, R.a);
         if (lane < 2 * rx) put(w, SLICE + lane, R.b);
     };
+    // tiles: the loads of a row (all lines, or only the last), and their LDS stores
+    constexpr int TSITES = TILE_L * TILE_W; // sites of the four lines: lanes take site lane and lane + 64
+    // the lane index, computed where it is used: the site and line numbers derived from it would
+    // otherwise be kept in registers through the entry loop, which costs a wavefront per SIMD
+    auto lane_now = []() {
+        int l;
+        __asm__ volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+        return l;
+    };
+    auto fetch_t = [&](RowRecs &R, int dl, bool one) {
+        const int lane = lane_now();
+        const uint32_t sb = (uint32_t)(t0 + dl - TILE_RX);
+        // record a: line TILE_L - 1, site lane (only the last line), or site lane of the four lines
+        const int ja = one ? TILE_L - 1 : lane / TILE_W, ia = one ? lane : lane % TILE_W;
+        if (!one || lane < TILE_W) R.a = bload_rec(rp, (sb + (uint32_t)(ja * tnx + ia)) * (uint32_t)sizeof(NodeRec));
+        // record b: site lane + 64 of the four lines
+        const int s2 = lane + SLICE;
+        if (!one && s2 < TSITES)
+            R.b = bload_rec(rp, (sb + (uint32_t)(s2 / TILE_W * tnx + s2 % TILE_W)) * (uint32_t)sizeof(NodeRec));
+    };
+    int sh = 0; // tiles: LDS line slot of the current row's line 0
+    auto stage_t = [&](const RowRecs &R, bool one) {
+        const int lane = lane_now();
+        double *w = &s_row[wv][0];
+        constexpr int LW = 4 * TILE_W; // doubles per line slot
+        const int ja = one ? TILE_L - 1 : lane / TILE_W, ia = one ? lane : lane % TILE_W;
+        if (!one || lane < TILE_W) put(w + ((sh + ja) & 3) * LW, ia, R.a);
+        const int s2 = lane + SLICE;
+        if (!one && s2 < TSITES) put(w + ((sh + s2 / TILE_W) & 3) * LW, s2 % TILE_W, R.b);
+    };
+    // the lane's first site in the row buffer (site x + TILE_RX of its line, for tiles)
+    auto row_base = [&]() -> const double * {
+        if constexpr (TILE) {
+            const int lane = lane_now();
+            return &s_row[wv][0] + ((sh + lane / TILE_X) & 3) * (4 * TILE_W) + lane % TILE_X + TILE_RX;
+        }
+        else
+            return &s_row[wv][0] + lane + rx;
+    };
+    const double *nbase = row_base();
     double f0 = 0, f1 = 0, f2 = 0;
     const StOff *sst = P.sst + RG.sst0;
-    const unsigned long long *mks = P.smaskT + eslice / SLICE;
+    // lane masks per (slice, offset), or per (tile, offset) for tiles
+    const unsigned long long *mks = TILE ? P.smaskT + RG.e0 / SLICE + (long long)tile * noff : P.smaskT + eslice / SLICE;
     // the slice's history words (noff <= 128): an offset whose bit is clear holds +0 in every entry,
     // which is taken without a load.  This kernel sets bits of them (atomicOr below), so they are read
     // once, here, with an ordinary global load (every lane the same address) made wavefront-uniform
     // with readfirstlane, not through the constant address space; the bits set later are for offsets
     // this wavefront has already passed.
-    unsigned long long *hzp = P.snz + 2 * (n0 / SLICE);
+    unsigned long long *hzp = P.snz + 2 * (TILE ? base / SLICE + tile : n0 / SLICE);
     auto uniform64 = [](unsigned long long w) {
         const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)w);
         const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(w >> 32));
@@ -727,9 +785,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         if (!bit) return;
         // the entry's byte offset in the slice's history (and in the parameter streams)
         const uint32_t so = hso + (uint32_t)k * (SLICE * TW);
-        const double *nb = &s_row[wv][0] + (lane + rx + d.st.dx);
-        const double nbx = nb[0], nby = nb[ST_ROW], nbz = nb[2 * ST_ROW];
-        const int key = krow + *reinterpret_cast<const int *>(nb + 3 * ST_ROW);
+        const double *nb = nbase + d.st.dx;
+        const double nbx = nb[0], nby = nb[RS], nbz = nb[2 * RS];
+        const int key = krow + *reinterpret_cast<const int *>(nb + 3 * RS);
         // the entry's key-table values: c (c * V in the mixed form), s0, sc
         const T *tk = PV == PV_GENERAL ? tab + key : lds_ptr<T>((uint32_t)key);
         constexpr uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
@@ -874,21 +932,52 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             a.z = fmaf(p, y2, a.z);
         }
     };
-    if (noff > 0) {
+    if (noff > 0 && wave_on) {
         RowRecs R;
-        fetch(R, sload(rows, 1));
-        stage(R);
+        int dfetch = sload(rows, 1); // slot delta of the row in flight
+        bool one = false;            // tiles: the row in flight loads only its last line
+        if constexpr (TILE) {
+            fetch_t(R, dfetch, false);
+            stage_t(R, false);
+            nbase = row_base();
+        } else {
+            fetch(R, dfetch);
+            stage(R);
+        }
         int r = 0, rend = sload(rows, 0);
-        if (nrow > 1) fetch(R, sload(rows, 3));
+        if (nrow > 1) {
+            const int d1 = sload(rows, 3);
+            if constexpr (TILE) {
+                one = d1 - dfetch == tnx;
+                fetch_t(R, d1, one);
+            } else {
+                fetch(R, d1);
+            }
+            dfetch = d1;
+        }
         // the next row's offset count and the slot delta of the row after it, read a row ahead
         int cnext = sload(rows, 2), dnext = sload(rows, 5);
         // entering offset o: at a row's end, stage the next row and start loading the one after
         auto next_row = [&](int o) {
             if (o == rend) {
                 r++;
-                stage(R);
+                if constexpr (TILE) {
+                    if (one) sh++;
+                    stage_t(R, one);
+                    nbase = row_base();
+                } else {
+                    stage(R);
+                }
                 rend += cnext;
-                if (r + 1 < nrow) fetch(R, dnext);
+                if (r + 1 < nrow) {
+                    if constexpr (TILE) {
+                        one = dnext - dfetch == tnx;
+                        fetch_t(R, dnext, one);
+                    } else {
+                        fetch(R, dnext);
+                    }
+                    dfetch = dnext;
+                }
                 cnext = sload(rows, 2 * (r + 1));
                 dnext = sload(rows, 2 * (r + 2) + 1);
             }
@@ -1537,16 +1626,16 @@ KernelFn pick_law(int law, bool lat)
 // launch variants of the stencil bond kernel: (minimum waves per SIMD, block order); orders: XCD
 // remap, launch order, and launch order over the stripe list (variants 4 and 5)
 
-template <bool MIXED, int LAW, int PV>
+template <bool MIXED, int LAW, int PV, bool TILE>
 KernelFn pick_occ(int v)
 {
     switch (v) {
-    case 1: return k_bond_st<MIXED, LAW, 1, false, PV>;
-    case 2: return k_bond_st<MIXED, LAW, 5, true, PV>;
-    case 3: return k_bond_st<MIXED, LAW, 5, false, PV>;
-    case 4: return k_bond_st<MIXED, LAW, 1, false, PV>;
-    case 5: return k_bond_st<MIXED, LAW, 5, false, PV>;
-    default: return k_bond_st<MIXED, LAW, 1, true, PV>;
+    case 1: return k_bond_st<MIXED, LAW, 1, false, PV, TILE>;
+    case 2: return k_bond_st<MIXED, LAW, 5, true, PV, TILE>;
+    case 3: return k_bond_st<MIXED, LAW, 5, false, PV, TILE>;
+    case 4: return k_bond_st<MIXED, LAW, 1, false, PV, TILE>;
+    case 5: return k_bond_st<MIXED, LAW, 5, false, PV, TILE>;
+    default: return k_bond_st<MIXED, LAW, 1, true, PV, TILE>;
     }
 }
 
@@ -1587,24 +1676,24 @@ void launch_bond(bbpd_solver *s, KernelFn kb, int v)
     }
 }
 
-template <bool MIXED, int PV>
+template <bool MIXED, int PV, bool TILE>
 KernelFn pick_law_st(int law, int v)
 {
     switch (law) {
-    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR, PV>(v);
-    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR, PV>(v);
-    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR, PV>(v);
-    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL, PV>(v);
+    case BBPD_LAW_LINEAR: return pick_occ<MIXED, BBPD_LAW_LINEAR, PV, TILE>(v);
+    case BBPD_LAW_BILINEAR: return pick_occ<MIXED, BBPD_LAW_BILINEAR, PV, TILE>(v);
+    case BBPD_LAW_TRILINEAR: return pick_occ<MIXED, BBPD_LAW_TRILINEAR, PV, TILE>(v);
+    default: return pick_occ<MIXED, BBPD_LAW_EXPONENTIAL, PV, TILE>(v);
     }
 }
 
-template <bool MIXED>
+template <bool MIXED, bool TILE>
 KernelFn pick_pv_st(int pv, int law, int v)
 {
     switch (pv) {
-    case PV_SCALAR: return pick_law_st<MIXED, PV_SCALAR>(law, v);
-    case PV_TABLE: return pick_law_st<MIXED, PV_TABLE>(law, v);
-    default: return pick_law_st<MIXED, PV_GENERAL>(law, v);
+    case PV_SCALAR: return pick_law_st<MIXED, PV_SCALAR, TILE>(law, v);
+    case PV_TABLE: return pick_law_st<MIXED, PV_TABLE, TILE>(law, v);
+    default: return pick_law_st<MIXED, PV_GENERAL, TILE>(law, v);
     }
 }
 
@@ -1616,7 +1705,10 @@ KernelFn pick_bond_kernel(const bbpd_solver *s, int variant)
         const int m0 = s->P.s0mode, mc = s->P.scmode;
         const int pv = (m0 == PM_SCALAR && mc == PM_SCALAR) ? PV_SCALAR
                        : (m0 == PM_TABLE && mc == PM_TABLE) ? PV_TABLE : PV_GENERAL;
-        return mixed ? pick_pv_st<true>(pv, s->law, v) : pick_pv_st<false>(pv, s->law, v);
+        // tiles are built for the mixed form only (the FP64 form keeps the slices' reduction order,
+        // which the oracle's device-order sums follow)
+        if (mixed) return s->tiles ? pick_pv_st<true, true>(pv, s->law, v) : pick_pv_st<true, false>(pv, s->law, v);
+        return pick_pv_st<false, false>(pv, s->law, v);
     }
     return mixed ? pick_law<true>(s->law, s->lattice) : pick_law<false>(s->law, s->lattice);
 }

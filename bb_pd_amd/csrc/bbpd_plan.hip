@@ -146,6 +146,19 @@ std::vector<unsigned long long> lane_masks(const std::vector<uint4> &m, long lon
     return r;
 }
 
+// The same per tile (TILE_X x TILE_L slots, tile_slot): bit l of (tile, offset) set when the slot of
+// lane l has the bond at that offset.
+std::vector<unsigned long long> tile_lane_masks(const std::vector<uint4> &m, long long ntiles, int noff, int nx)
+{
+    std::vector<unsigned long long> r((size_t)ntiles * noff, 0ull);
+    for (long long t = 0; t < ntiles; t++)
+        for (int l = 0; l < SLICE; l++) {
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(&m[tile_slot((int)t, l, nx)]);
+            for (int o = 0; o < noff; o++)
+                if ((w[o >> 5] >> (o & 31)) & 1u) r[(size_t)t * noff + o] |= 1ull << l;
+        }
+    return r;
+}
 
 // Per-offset records of the stencil layout, then ST_PAD copies of the last.
 // Mixed form: xi = h * d rounded to FP32 per component (an offset and its negation give exactly

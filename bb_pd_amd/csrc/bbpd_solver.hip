@@ -76,6 +76,19 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     s->stencil = sp.ok;
     s->layout = sp.ok ? "stencil" : "records (" + sp.why + ")";
     if (s->stencil && env_launch_variant() >= 0) s->occ = env_launch_variant();
+    // tiles of TILE_X x TILE_L nodes (mixed form), asked for with BBPD_TILES=1: the box's length along x
+    // a multiple of TILE_X, its lines in groups of TILE_L, offsets within TILE_RX along x, the entry
+    // offsets within 31 bits.  Slices are the default: on the 400 x 40 x 80 beam the tiled kernel stages
+    // 46% fewer sites but spends more on its line bookkeeping, and runs slower (206 against 181-192 us).
+    const int tnx = sp.ok ? sp.kmax[0] - sp.kmin[0] + 1 : 0;
+    {
+        const char *te = std::getenv("BBPD_TILES");
+        const bool on = te && std::string(te) == "1";
+        const long long nS = ((long long)nN + SLOT_ALIGN - 1) / SLOT_ALIGN * SLOT_ALIGN;
+        s->tiles = sp.ok && mixed && on && tnx % TILE_X == 0 && (nN / tnx) % TILE_L == 0 && sp.rx <= TILE_RX &&
+                   (nS * sp.noff + 2 * SLICE) * (long long)sizeof(float) < (1LL << 31);
+        if (s->tiles) s->layout = "stencil, 16x4 tiles";
+    }
 
     // ---- slots.  Stencil layout: the x-fastest lattice order.  Records: tiles of 64 x 2 x 2 nodes (64 along the longest axis) visited in Morton order,
     // lines of 64 inside a tile, so a wavefront is 64 consecutive nodes of a lattice line and the
@@ -362,14 +375,16 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
             setbit(s->slot_of[bj[k]], sp.oj[k]);
         }
         BBPD_HIP(hipMemcpy(maskd, hm.data(), sizeof(uint4) * nSlots, hipMemcpyHostToDevice));
-        const std::vector<unsigned long long> hmt = lane_masks(hm, nSlots, sp.noff);
+        const std::vector<unsigned long long> hmt =
+            s->tiles ? tile_lane_masks(hm, nN / SLICE, sp.noff, tnx) : lane_masks(hm, nSlots, sp.noff);
         BBPD_HIP(hipMemcpy(maskTd, hmt.data(), sizeof(unsigned long long) * hmt.size(), hipMemcpyHostToDevice));
         if (mixed) {
             std::vector<float> h0(nE, 0.0f);
             for (int k = 0; k < nB; k++)
                 if (pr->fail && pr->fail[k] == 0) h0[s->entry_i[k]] = h0[s->entry_j[k]] = -0.0f;
             BBPD_HIP(hipMemcpy(hsd, h0.data(), sizeof(float) * nE, hipMemcpyHostToDevice));
-            const std::vector<unsigned long long> hw = history_words(h0.data(), s->nSlices, sp.noff);
+            const std::vector<unsigned long long> hw = s->tiles ? tile_history_words(h0.data(), nN / SLICE, sp.noff, tnx)
+                                                                : history_words(h0.data(), s->nSlices, sp.noff);
             BBPD_HIP(hipMemcpy(snzd, hw.data(), sizeof(unsigned long long) * hw.size(), hipMemcpyHostToDevice));
         } else {
             std::vector<double> h0(nE, 0.0);
@@ -491,6 +506,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     R0.nrow = sp.ok ? sp.nrow : 0;
     R0.row0 = 0;
     R0.rx = sp.rx;
+    R0.tnx = s->tiles ? tnx : 0;
+    R0.ntile = s->tiles ? nN / SLICE : 0;
     R0.sst0 = 0;
     R0.fast64 = fast64;
     R0.kg = sp.G;
@@ -832,7 +849,9 @@ int fetch_hs(bbpd_solver *s, std::vector<double> &hv)
 int put_hs(bbpd_solver *s, const std::vector<double> &hv)
 {
     const long long nE = s->nPadded;
-    const std::vector<unsigned long long> hw = history_words(hv.data(), s->nSlices, (int)(nE / s->nSlices / SLICE));
+    const int noff = (int)(nE / s->nSlices / SLICE);
+    const std::vector<unsigned long long> hw = s->tiles ? tile_history_words(hv.data(), s->reg[0].ntile, noff, s->reg[0].tnx)
+                                                        : history_words(hv.data(), s->nSlices, noff);
     BBPD_HIP(hipMemcpy(s->P.snz, hw.data(), sizeof(unsigned long long) * hw.size(), hipMemcpyHostToDevice));
     if (s->precision == BBPD_PRECISION_MIXED) {
         std::vector<float> f(nE);
@@ -1248,7 +1267,9 @@ extern "C" int bbpd_solver_bond_bytes(bbpd_solver *s, int64_t *bytes, int64_t *o
     for (size_t r = 0; r < s->reg.size(); r++) {
         if (r >= s->active.size() || !s->active[r]) continue;
         const Region &RG = s->reg[r];
-        const int64_t sl0 = RG.base / SLICE, sl1 = (RG.base + ((int64_t)RG.nN + SLICE - 1) / SLICE * SLICE) / SLICE;
+        // words per slice, or per tile (a region's tiles take its first words)
+        const int64_t sl0 = RG.base / SLICE,
+                      sl1 = RG.ntile ? sl0 + RG.ntile : (RG.base + ((int64_t)RG.nN + SLICE - 1) / SLICE * SLICE) / SLICE;
         for (int64_t q = sl0; q < sl1; q++) set += __builtin_popcountll(hw[2 * q]) + __builtin_popcountll(hw[2 * q + 1]);
         total += (sl1 - sl0) * RG.noff;
     }

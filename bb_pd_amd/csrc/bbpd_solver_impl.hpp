@@ -37,6 +37,13 @@ constexpr int ST_MAX = 128;    // offsets of the stencil layout (bits of the per
 constexpr int ST_GMAX = 45;    // distinct family sizes of the stencil layout (key table <= 45^2)
 constexpr int ST_RMAX = 8;     // largest |dx| of a stencil offset
 constexpr int ST_ROW = 80;     // sites of a staged stencil row: 64 + 2 * ST_RMAX
+// Stencil tiles (mixed form): a wavefront takes 16 consecutive slots (x) on each of 4 lattice lines
+// (slot stride = the box's length along x), and stages each stencil row as 4 lines of TILE_W sites
+// through a window that slides one line per step in dy
+constexpr int TILE_X = 16;
+constexpr int TILE_L = 4;
+constexpr int TILE_RX = 3;     // largest |dx| the tiles take
+constexpr int TILE_W = TILE_X + 2 * TILE_RX; // sites of a staged line
 // node-flag bits 0-2: DOF d constrained; bit 3: member of a contact family (its DOFs are driven by
 // the rigid-body projection, so the ADR Rayleigh quotient leaves the node out)
 constexpr unsigned char CONTACT_BIT = 1u << 3;
@@ -61,6 +68,8 @@ struct Region {
     long long e0;                  // stencil layout: first entry of the region
     int noff, off0;                // stencil layout: offsets off0 .. off0 + noff - 1
     int nrow, row0, rx;            // stencil layout: rows row0 .. row0 + nrow - 1, largest |dx|
+    int tnx, ntile;                // stencil layout in tiles (0 = slices): the box's length along x
+                                   // and the tile count (see TILE_X)
     int sst0;                      // stencil layout: first per-offset record
     int kg, tab0, ntab, ts0, tsc;  // stencil layout: key tables (ts0 / tsc relative to tab0)
     int fast64;                    // stencil layout, FP64 form: undeformed lengths, their reciprocals
@@ -277,6 +286,7 @@ struct bbpd_solver {
     int graph_steps = 0;
     int graph_version = -1;
     int occ = 0; // launch variant of the stencil bond kernel (pick_occ)
+    bool tiles = false; // the stencil bond kernel runs in tiles of TILE_X x TILE_L nodes
 };
 
 namespace bbpd_sv {
@@ -326,6 +336,32 @@ std::vector<unsigned long long> history_words(const V *h, long long nsl, int nof
     return w;
 }
 
+// Stencil tiles: the slot of lane l of tile t in a box of nx sites per line (TILE_X x TILE_L lanes;
+// tiles run along x, then over groups of TILE_L lines)
+__host__ __device__ inline int tile_slot(int t, int l, int nx)
+{
+    const int tx = t % (nx / TILE_X), g = t / (nx / TILE_X);
+    return TILE_X * tx + (l % TILE_X) + nx * (TILE_L * g + l / TILE_X);
+}
+
+// The history words of `nt` tiles (as history_words, per tile): bit o set when an entry of offset o
+// of a lane of the tile holds a history word other than +0; the history `h` is in the slice order
+// (slice, offset, lane) of the slots.
+template <typename V>
+std::vector<unsigned long long> tile_history_words(const V *h, long long nt, int noff, int nx)
+{
+    std::vector<unsigned long long> w((size_t)nt * 2, 0ull);
+    for (long long t = 0; t < nt; t++)
+        for (int l = 0; l < SLICE; l++) {
+            const long long q = tile_slot((int)t, l, nx);
+            const V *e = h + (q / SLICE) * noff * SLICE + q % SLICE;
+            for (int o = 0; o < noff; o++)
+                if (e[(long long)o * SLICE] != 0 || std::signbit(e[(long long)o * SLICE]))
+                    w[(size_t)t * 2 + (o >> 6)] |= 1ull << (o & 63);
+        }
+    return w;
+}
+
 // ---- bbpd_kernels.hip: kernel choice and launches
 std::vector<int> stripe_list(const std::vector<int> &list);
 int env_launch_variant();
@@ -344,6 +380,7 @@ bool make_table(const double *a, int n, int cap, std::vector<double> &tab, std::
 int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj);
 bool detect_lattice(const bbpd_problem *pr, double hlat[3], double xmin[3], double xmax[3], double &hq);
 std::vector<unsigned long long> lane_masks(const std::vector<uint4> &m, long long nslots, int noff);
+std::vector<unsigned long long> tile_lane_masks(const std::vector<uint4> &m, long long ntiles, int noff, int nx);
 std::vector<StOff> offset_records(const std::vector<int4> &off, const double *h, double hmr, double hpr,
                                   double horizon, double two_r, bool mixed, const int *kmin, const int *kmax,
                                   int *fast64);
