@@ -279,25 +279,8 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     BBPD_REQUIRE(pr->schedule != 2 || (pr->integrator == BBPD_INTEGRATOR_ADR && pr->eqTol >= 0 &&
                                        pr->itersPerIncrementMin >= 0),
                  "the equilibrium staircase needs ADR, eqTol >= 0 and itersPerIncrementMin >= 0");
-    BBPD_HIP(hipSetDevice(s->device));
-    int rc;
-    std::vector<int> bi, bj;
-    if ((rc = check_bonds(pr, bi, bj))) return rc;
-    double xmin[3], xmax[3], hq, hlat[3];
-    if (!detect_lattice(pr, hlat, xmin, xmax, hq) || !same_bits(hlat[0], s->DX) || !same_bits(hlat[1], s->DX) ||
-        !same_bits(hlat[2], s->DX)) {
-        bbpd::set_error("a batch problem must be a lattice of spacing DX (X == DX * k)");
-        return BBPD_ERR_UNSUPPORTED;
-    }
-    StencilPlan sp;
-    plan_stencil(pr, pr->undeformedCoordinates, bi, bj, hlat, sp);
-    if (!sp.ok) {
-        bbpd::set_error("a batch problem needs the stencil layout: " + sp.why);
-        return BBPD_ERR_UNSUPPORTED;
-    }
-    int famtot = 0;
-    for (int b = 0; b < pr->nBodies; b++) famtot += pr->nFamily[b];
-    BBPD_REQUIRE(famtot <= RB_FAM, "contact families too large for a batch region");
+    // room first (a region and a free slot range), before the set-up checks and the plan: a caller
+    // that retries while the batch is full pays only this test
     // a problem larger than the whole slot pool can never be admitted: a distinct error, so that a
     // caller does not wait for slots that never free up
     if ((long long)(nN + BLOCK - 1) / BLOCK > (long long)s->freeblk.size()) {
@@ -327,6 +310,25 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
         bbpd::set_error(rid < 0 ? "batch full: no free region" : "batch full: no free slot range");
         return BBPD_ERR_FULL;
     }
+    BBPD_HIP(hipSetDevice(s->device));
+    int rc;
+    std::vector<int> bi, bj;
+    if ((rc = check_bonds(pr, bi, bj))) return rc;
+    double xmin[3], xmax[3], hq, hlat[3];
+    if (!detect_lattice(pr, hlat, xmin, xmax, hq) || !same_bits(hlat[0], s->DX) || !same_bits(hlat[1], s->DX) ||
+        !same_bits(hlat[2], s->DX)) {
+        bbpd::set_error("a batch problem must be a lattice of spacing DX (X == DX * k)");
+        return BBPD_ERR_UNSUPPORTED;
+    }
+    StencilPlan sp;
+    plan_stencil(pr, pr->undeformedCoordinates, bi, bj, hlat, sp);
+    if (!sp.ok) {
+        bbpd::set_error("a batch problem needs the stencil layout: " + sp.why);
+        return BBPD_ERR_UNSUPPORTED;
+    }
+    int famtot = 0;
+    for (int b = 0; b < pr->nBodies; b++) famtot += pr->nFamily[b];
+    BBPD_REQUIRE(famtot <= RB_FAM, "contact families too large for a batch region");
     const int base = blk0 * BLOCK, nSl = need * BLOCK;
     const size_t nS = (size_t)s->nSlots;
     const long long e0 = (long long)base * ST_MAX;
@@ -338,7 +340,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     // node data of the region's slots
     std::vector<NodeRec> hpos(nSl);
     std::memset(hpos.data(), 0, sizeof(NodeRec) * nSl);
-    std::vector<double> hX(3 * (size_t)nSl, 0.0), zero(3 * (size_t)nSl, 0.0), hrho(nSl, 1.0), hm(nSl, 1.0);
+    std::vector<double> hX(3 * (size_t)nSl, 0.0), hrho(nSl, 1.0), hm(nSl, 1.0);
     std::vector<unsigned char> hc(nSl, 0);
     std::vector<uint4> hmask(nSl, make_uint4(0, 0, 0, 0));
     auto &RH = s->rh[rid];
@@ -362,9 +364,11 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
         hrho[q] = pr->DENSITY[n];
         if (pr->massVector) hm[q] = pr->massVector[n];
     }
-    // entries: masks, history (sign bit set for bonds that start failed), streamed parameters
+    // entries: masks, history (sign bit set for bonds that start failed; all +0 when none does, which
+    // is set on the device), streamed parameters
     const long long nEr = (long long)nSl * noff;
-    std::vector<double> h0(nEr, 0.0), hs0(pr->s0 ? nEr : 0, 0.0), hsc(pr->sc ? nEr : 0, 0.0);
+    const bool any_failed = pr->fail && std::any_of(pr->fail, pr->fail + nB, [](double f) { return f == 0; });
+    std::vector<double> h0(any_failed ? nEr : 0, 0.0), hs0(pr->s0 ? nEr : 0, 0.0), hsc(pr->sc ? nEr : 0, 0.0);
     RH.entry_i.assign(nB, 0);
     RH.entry_j.assign(nB, 0);
     auto dense_local = [&](int q, int o) { return ((long long)(q / SLICE) * noff + o) * SLICE + (q % SLICE); };
@@ -375,7 +379,7 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
         RH.entry_j[k] = e0 + ej;
         reinterpret_cast<uint32_t *>(&hmask[qi])[sp.oi[k] >> 5] |= 1u << (sp.oi[k] & 31);
         reinterpret_cast<uint32_t *>(&hmask[qj])[sp.oj[k] >> 5] |= 1u << (sp.oj[k] & 31);
-        if (pr->fail && pr->fail[k] == 0) h0[ei] = h0[ej] = -0.0;
+        if (any_failed && pr->fail[k] == 0) h0[ei] = h0[ej] = -0.0;
         if (pr->s0) hs0[ei] = hs0[ej] = pr->s0[k];
         if (pr->sc) hsc[ei] = hsc[ej] = pr->sc[k];
     }
@@ -485,10 +489,11 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     double *fpc[3] = {P.Fp0, P.Fp1, P.Fp2};
     for (int d = 0; d < 3; d++) {
         BBPD_HIP(hipMemcpy(Xc[d] + base, hX.data() + (size_t)nSl * d, sizeof(double) * nSl, hipMemcpyHostToDevice));
-        if (uc[d]) BBPD_HIP(hipMemcpy(uc[d] + base, zero.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
-        BBPD_HIP(hipMemcpy(vc[d] + base, zero.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
-        BBPD_HIP(hipMemcpy(fc[d] + base, zero.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
-        if (fpc[d]) BBPD_HIP(hipMemcpy(fpc[d] + base, zero.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
+        // zero fills on the batch's stream (outside any captured graph), ordered before its next steps
+        if (uc[d]) BBPD_HIP(hipMemsetAsync(uc[d] + base, 0, sizeof(double) * nSl, s->stream));
+        BBPD_HIP(hipMemsetAsync(vc[d] + base, 0, sizeof(double) * nSl, s->stream));
+        BBPD_HIP(hipMemsetAsync(fc[d] + base, 0, sizeof(double) * nSl, s->stream));
+        if (fpc[d]) BBPD_HIP(hipMemsetAsync(fpc[d] + base, 0, sizeof(double) * nSl, s->stream));
     }
     BBPD_HIP(hipMemcpy((double *)P.rho + base, hrho.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
     if (P.mass) BBPD_HIP(hipMemcpy((double *)P.mass + base, hm.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
@@ -509,11 +514,16 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
         }
         return BBPD_OK;
     };
-    if ((rc = put_entries(P.hs, h0))) return rc;
-    {
+    if (any_failed) {
+        if ((rc = put_entries(P.hs, h0))) return rc;
         const std::vector<unsigned long long> hw = history_words(h0.data(), nSl / SLICE, noff);
         BBPD_HIP(hipMemcpy(P.snz + 2 * (base / SLICE), hw.data(), sizeof(unsigned long long) * hw.size(),
                            hipMemcpyHostToDevice));
+    } else {
+        BBPD_HIP(hipMemsetAsync((char *)P.hs + (size_t)e0 * (mixed ? 4 : 8), 0, (size_t)nEr * (mixed ? 4 : 8),
+                                s->stream));
+        BBPD_HIP(hipMemsetAsync(P.snz + 2 * (base / SLICE), 0, sizeof(unsigned long long) * 2 * (nSl / SLICE),
+                                s->stream));
     }
     if (pr->s0 && (rc = put_entries((void *)P.s0str, hs0))) return rc;
     if (pr->sc && (rc = put_entries((void *)P.scstr, hsc))) return rc;

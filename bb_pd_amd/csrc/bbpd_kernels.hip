@@ -569,10 +569,20 @@ __device__ __forceinline__ bool law_active(T s, T thr, bool flag)
     return s > thr;
 }
 
-// Mixed form, stencil layout: the stretch by its Taylor series in x = (|y|^2 - |xi|^2) / (2 |xi|^2)
-// for |x| at most this, else by a reciprocal square root and a reciprocal.  Both round to within a
-// few ulps of the exact stretch.
-constexpr float MIX_SERIES_X = 0x1p-9f;
+// Mixed form, stencil layout: with x = (|y|^2 - |xi|^2) / (2 |xi|^2), the stretch is
+// s = sqrt(1 + 2x) - 1 and the force factor g = s / (1 + s) = 1 - (1 + 2x)^(-1/2).  For |x| at most
+// this both come from their series in x, else from a reciprocal square root and a reciprocal.  Both
+// round to within a few ulps of the exact values.
+constexpr float MIX_SERIES_X = 0x1p-10f;
+
+// Mixed form: a law threshold on the stretch mapped to x = s + s^2 / 2 (increasing in s), lowered by a
+// relative 2^-20 so that the test on x runs the law for every entry whose stretch passes the threshold
+// (and for a few at it, where the law changes nothing)
+__device__ __forceinline__ float thr_to_x(float t)
+{
+    const float x = fmaf(0.5f * t, t, t);
+    return x - fabsf(x) * 0x1p-20f;
+}
 
 // Launch variants, picked per solver by an autotuner: OCC is the minimum waves per SIMD the compiler
 // must allow (1 = its own choice), which bounds the registers per wave, trading instruction-level
@@ -607,8 +617,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
     const int ts0 = RG.ts0, tsc = RG.tsc;
     const bool fast64 = !MIXED && RG.fast64;
     const bool real = TILE ? wave_on : n - base < RG.nN;
-    const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = law_threshold<LAW, T>(rs0, rsc, rs1);
+    // the law's threshold: on the stretch (FP64 form), on x (mixed form, thr_to_x)
+    auto thr_of = [](T t) -> T {
+        if constexpr (MIXED) return thr_to_x(t);
+        else return t;
+    };
+    const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = thr_of(law_threshold<LAW, T>(rs0, rsc, rs1));
     const T rs0lo = (T)RG.s0lo, rsclo = (T)RG.sclo;
+    // PV_GENERAL without key tables for s0 / sc: region scalars, a streamed value entering as the
+    // region's smallest (see below)
+    const bool gtab = P.s0mode == PM_TABLE || P.scmode == PM_TABLE;
+    const T gs0 = P.s0mode == PM_STREAM ? rs0lo : rs0, gsc = P.scmode == PM_STREAM ? rsclo : rsc;
+    const T gthr = thr_of(law_threshold<LAW, T>(gs0, gsc, rs1));
     const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
     const int cur = (int)(*P.tick & 1);
     const NodeRec *__restrict__ pos = P.pos[cur];
@@ -620,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
     if (PV == PV_TABLE) {
         for (int i = threadIdx.x; i < kg * kg; i += BLOCK) {
             tdst[2 * i] = tsrc[tab0 + i];
-            tdst[2 * i + 1] = law_threshold<LAW, T>(tsrc[tab0 + ts0 + i], tsrc[tab0 + tsc + i], (T)RG.s1);
+            tdst[2 * i + 1] = thr_of(law_threshold<LAW, T>(tsrc[tab0 + ts0 + i], tsrc[tab0 + tsc + i], (T)RG.s1));
         }
     } else {
         for (int i = threadIdx.x; i < ntab; i += BLOCK) tdst[i] = tsrc[tab0 + i];
@@ -808,13 +828,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
             // a streamed s0 / sc enters the threshold as the region's smallest value: a stretch at or
             // below it cannot change the law's state, and the entry's own value is read only when the
             // law runs (most entries, most steps, read no parameter stream)
-            s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : (P.s0mode == PM_STREAM ? rs0lo : rs0);
-            sc = P.scmode == PM_TABLE ? tab[tsc + key] : (P.scmode == PM_STREAM ? rsclo : rsc);
-            thr = law_threshold<LAW, T>(s0, sc, rs1);
+            s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : gs0;
+            sc = P.scmode == PM_TABLE ? tab[tsc + key] : gsc;
+            thr = gtab ? thr_of(law_threshold<LAW, T>(s0, sc, rs1)) : gthr;
         }
-        // the law, and the store of a changed history (sign = flag)
-        auto law = [&](T s) {
-            if (!law_active<LAW, T>(s, thr, flag)) return;
+        // the law at stretch s once its test has passed, and the store of a changed history (sign = flag)
+        auto law_run = [&](T s) {
             if (PV == PV_TABLE) {
                 const uint32_t ki = (uint32_t)(key - tabl) / KSCALE;
                 s0 = bload<T>(rtab, (uint32_t)(ts0 + ki) * TW);
@@ -835,6 +854,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
             }
         };
         if constexpr (!MIXED) {
+            auto law = [&](double s) {
+                if (law_active<LAW, T>(s, thr, flag)) law_run(s);
+            };
             const double Xm0 = P.hx * (double)(kx + d.st.dx);
             // calculatedeformedlengthMex.c:37-43, row-oriented (x_m - x_n); l and 1 / l by the unscaled
             // sequences when every lane's squared length lies in their range (wave-uniform branch)
@@ -907,22 +929,33 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
             const float num = fmaf(et0, fmaf(2.0f, d.st.x, et0),
                                    fmaf(et1, fmaf(2.0f, d.st.y, et1), et2 * fmaf(2.0f, d.st.z, et2)));
             const float x = num * d.st.rq;
-            // |x| <= 2^-9: Taylor series, s = x - x^2/2 + x^3/2 (the next term, -5 x^4 / 8, is below
-            // 2^-27 |s|) and g = s - s^2 + s^3 (next s^4, below 2^-26 |g|).  The choice is the
-            // bond's own (x is the same from either end), so both ends store the same history.
-            float s = x * fmaf(x, fmaf(x, 0.5f, -0.5f), 1.0f);
-            float g = fmaf(s, fmaf(s, s, -s), s);
-            if (!(fabsf(x) <= MIX_SERIES_X)) {
-                // |y|^2 carries 1e-30 (below every representable bond's square by many orders, so it
-                // changes no realistic length): 1/l stays finite for a bond of zero length, whose
-                // force p * y is then 0, the reference's NaN -> 0 (calculatebondforcesMex.c:38)
+            // |x| <= 2^-10: series, g = x - 3/2 x^2 + 5/2 x^3 (the next term, -35/8 x^4, is below 2^-27
+            // |g|), and, only where the law runs, s = x - x^2/2 + x^3/2 (next -5/8 x^4, below 2^-30
+            // |s|).  The law's test compares x with its threshold mapped to x.  The choice is the bond's
+            // own (x is the same from either end), so both ends store the same history.
+            float g = x * fmaf(x, fmaf(x, 2.5f, -1.5f), 1.0f);
+            // off the series: |y|^2 carries 1e-30 (below every representable bond's square by many
+            // orders, so it changes no realistic length): 1/l stays finite for a bond of zero length,
+            // whose force p * y is then 0, the reference's NaN -> 0 (calculatebondforcesMex.c:38)
+            auto off_series = [&](float &rl) {
                 const float l2 = fmaf(y0, y0, fmaf(y1, y1, fmaf(y2, y2, 1e-30f)));
-                const float rl = __builtin_amdgcn_rsqf(l2);
+                rl = __builtin_amdgcn_rsqf(l2);
                 const float l = l2 * rl;
-                s = num * __builtin_amdgcn_rcpf(d.st.L0 * (l + d.st.L0));
-                g = (s * rl) * d.st.L0;
+                return num * __builtin_amdgcn_rcpf(d.st.L0 * (l + d.st.L0)); // the stretch
+            };
+            if (!(fabsf(x) <= MIX_SERIES_X)) {
+                float rl;
+                const float so = off_series(rl);
+                g = (so * rl) * d.st.L0;
             }
-            law(s);
+            // the stretch, where the law runs: its series, or the same sequence as above
+            if (law_active<LAW, T>(x, thr, flag)) {
+                float rl, sl;
+                if ((__float_as_uint(x) & 0x7fffffffu) <= __float_as_uint(MIX_SERIES_X))
+                    sl = x * fmaf(x, fmaf(x, 0.5f, -0.5f), 1.0f);
+                else sl = off_series(rl);
+                law_run(sl);
+            }
             // kap (1 - H) s / l = (c vcf / |xi|) (1 - H) g; a failed bond of the brittle law carries
             // no force
             const float kap = c * d.st.vr;

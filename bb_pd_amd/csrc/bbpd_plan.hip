@@ -1,4 +1,6 @@
 // bbpd_plan.hip -- set-up checks of a problem and the stencil-layout plan (host code).
+#include <thread>
+
 #include "bbpd_solver_impl.hpp"
 
 namespace bbpd_sv {
@@ -49,19 +51,22 @@ bool make_table(const double *a, int n, int cap, std::vector<double> &tab, std::
 // Checks of a problem's bond arrays: ids, UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS against their
 // formulas (both are recomputed on the device: buildhorizons.m:93-97,
 // calculatevolumecorrectionfactors.m:20-32), fail and BONDTYPE values.  Fills the 0-based ends.
-int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj)
+namespace {
+
+// The checks of bonds [k0, k1) (see check_bonds): BBPD_OK, or the error code of the first that fails,
+// with the message set when `report`.
+int check_bond_range(const bbpd_problem *pr, int k0, int k1, int *bi, int *bj, bool report)
 {
     const int nN = pr->nNodes, nB = pr->nBonds;
     const double *X = pr->undeformedCoordinates, *BL = pr->BONDLIST;
     const double horizon = pr->horizon, RADIJ = pr->RADIJ;
     const double hmr = horizon - RADIJ, hpr = horizon + RADIJ, two_r = 2 * RADIJ;
-    bi.assign(nB, 0);
-    bj.assign(nB, 0);
-    for (int k = 0; k < nB; k++) {
+    for (int k = k0; k < k1; k++) {
         double di = BL[k], dj = BL[(size_t)nB + k];
         int i = (int)di, j = (int)dj;
         if (di != (double)i || dj != (double)j || i < 1 || j < 1 || i > nN || j > nN || i == j) {
-            bbpd::set_error("BONDLIST row " + std::to_string(k + 1) + " is not a pair of distinct ids in 1..nNodes");
+            if (report)
+                bbpd::set_error("BONDLIST row " + std::to_string(k + 1) + " is not a pair of distinct ids in 1..nNodes");
             return BBPD_ERR_ARG;
         }
         bi[k] = i - 1;
@@ -70,8 +75,9 @@ int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &
                ez = X[i - 1 + 2 * (size_t)nN] - X[j - 1 + 2 * (size_t)nN];
         double L0 = std::sqrt((ex * ex + ey * ey) + ez * ez);
         if (!bits_equal(L0, pr->UNDEFORMEDLENGTH[k])) {
-            bbpd::set_error("UNDEFORMEDLENGTH(" + std::to_string(k + 1) +
-                            ") differs from the length recomputed from undeformedCoordinates (buildhorizons.m:93-97)");
+            if (report)
+                bbpd::set_error("UNDEFORMEDLENGTH(" + std::to_string(k + 1) +
+                                ") differs from the length recomputed from undeformedCoordinates (buildhorizons.m:93-97)");
             return BBPD_ERR_UNSUPPORTED;
         }
         double vcf;
@@ -79,19 +85,52 @@ int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &
         else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
         else vcf = 0;
         if (!bits_equal(vcf, pr->VOLUMECORRECTIONFACTORS[k])) {
-            bbpd::set_error("VOLUMECORRECTIONFACTORS(" + std::to_string(k + 1) +
-                            ") differs from calculatevolumecorrectionfactors(L0, horizon, RADIJ)");
+            if (report)
+                bbpd::set_error("VOLUMECORRECTIONFACTORS(" + std::to_string(k + 1) +
+                                ") differs from calculatevolumecorrectionfactors(L0, horizon, RADIJ)");
             return BBPD_ERR_UNSUPPORTED;
         }
         if (pr->fail && !(pr->fail[k] == 0 || pr->fail[k] == 1)) {
-            bbpd::set_error("fail must hold 0 or 1");
+            if (report) bbpd::set_error("fail must hold 0 or 1");
             return BBPD_ERR_ARG;
         }
         if (pr->BONDTYPE && !(pr->BONDTYPE[k] == 0 || pr->BONDTYPE[k] == 1 || pr->BONDTYPE[k] == 2)) {
-            bbpd::set_error("BONDTYPE must hold 0, 1 or 2");
+            if (report) bbpd::set_error("BONDTYPE must hold 0, 1 or 2");
             return BBPD_ERR_ARG;
         }
     }
+    return BBPD_OK;
+}
+
+} // namespace
+
+// Checks of a problem's bond arrays: ids, UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS against their
+// formulas (both are recomputed on the device: buildhorizons.m:93-97,
+// calculatevolumecorrectionfactors.m:20-32), fail and BONDTYPE values.  Fills the 0-based ends.  Large
+// problems are checked in chunks by several host threads; the first failing bond in bond order is
+// the one reported.
+int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj)
+{
+    const int nB = pr->nBonds;
+    bi.assign(nB, 0);
+    bj.assign(nB, 0);
+    const int T = nB < (1 << 20) ? 1 : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    const int chunk = (nB + T - 1) / T;
+    std::vector<int> rc(T, BBPD_OK);
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++)
+        th.emplace_back([&, t] {
+            rc[t] = check_bond_range(pr, std::min(nB, t * chunk), std::min(nB, (t + 1) * chunk), bi.data(), bj.data(),
+                                     false);
+        });
+    rc[0] = check_bond_range(pr, 0, std::min(nB, chunk), bi.data(), bj.data(), true);
+    for (auto &x : th) x.join();
+    for (int t = 0; t < T; t++)
+        if (rc[t] != BBPD_OK) {
+            // the first failing chunk again, on this thread, for its message
+            return check_bond_range(pr, std::min(nB, t * chunk), std::min(nB, (t + 1) * chunk), bi.data(), bj.data(),
+                                    true);
+        }
     return BBPD_OK;
 }
 

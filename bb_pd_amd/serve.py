@@ -81,15 +81,18 @@ def slot_bytes(precision: int) -> int:
 def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: int = 200,
           increment_um: float = 25.0, stop_fraction: float = 0.2, stop_displacement: float = 5e-4,
           max_regions: int = 64, keep_history: bool = False, keep_cases: bool = False,
-          closed_loop: bool = False, lookahead: int = 2, log=None) -> dict:
+          closed_loop: bool = False, build_workers: int = 8, lookahead: int = 16, log=None) -> dict:
     """Run the service over `requests` (their `arrival` times are honoured unless `closed_loop`,
     in which case all are queued at t = 0).  A request that carries only its parameters has its
-    inputs generated after it arrives: the head of the queue when it is next to be admitted, and up
-    to `lookahead` queued requests on the host while the device runs the admitted requests' steps.
-    Input generation is therefore part of its time to solution.  Once admitted, its host arrays are
-    dropped (unless `keep_cases`): its state lives on the device until it is retired.  A request that
-    needs more node slots than the whole batch holds is rejected.  Returns per-request times and
+    inputs generated after it arrives, by a pool of `build_workers` host threads (the lattice
+    generator releases the interpreter lock), for at most `lookahead` queued requests at a time, so
+    input generation overlaps the device's steps and other requests' generation, and counts in the
+    request's time to solution.  Requests are admitted first come, first served, once their inputs
+    exist and the batch has a region and the node slots; once admitted, a request's host arrays are
+    dropped (unless `keep_cases`) and its state lives on the device until it is retired.  A request
+    that needs more node slots than the whole batch holds is rejected.  Returns per-request times and
     aggregate figures."""
+    import concurrent.futures as cf
     import torch
     slots = int(hbm_budget // slot_bytes(precision))
     pending = sorted(requests, key=lambda r: r.arrival)
@@ -97,77 +100,116 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
     active: dict = {}
     done: list = []
     rejected: list = []
+    futures: dict = {}
+    admit_s: list = []
+    build_s: list = []
+    # wall time launching device chunks (a graph is captured again when the membership changed), waiting
+    # for them, and waiting for a request's inputs
+    t_launch = t_sync = t_wait = 0.0
     bond_steps = 0
     # the batch's shared scalars (law, integrator, time step, spacing, horizon) from a request of
     # the same workload, built before the clock starts
     shape = pending[0].case if pending[0].case is not None else build_request(pending[0].params, ipi, increment_um)
+    pool = cf.ThreadPoolExecutor(max_workers=max(1, build_workers))
     with Batch(shape, precision, max_regions=max_regions, max_slots=slots, history_rows=1 << 17) as b:
         del shape
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
 
+        def build(r):
+            t = time.perf_counter()
+            case = build_request(r.params, ipi, increment_um)
+            return case, time.perf_counter() - t
+
         def arrivals():
             t = now()
             while pending and (closed_loop or pending[0].arrival <= t):
                 queue.append(pending.pop(0))
-
-        def build(r):
-            if r.case is None:
-                r.case = build_request(r.params, ipi, increment_um)
-                r.built = now()
-
-        while pending or queue or active:
-            arrivals()
-            # admission, first come first served; a request that does not fit waits for slots
-            while queue:
-                r = queue[0]
-                build(r)
-                try:
-                    r.region = b.admit(r.case, stop_fraction, stop_displacement)
-                except BatchFull:
-                    break
-                except BatchTooSmall as e:
-                    r.rejected = str(e)
-                    rejected.append(queue.pop(0))
-                    continue
-                r.admitted = now()
-                r.name, r.nBonds = r.case.name, r.case.nBonds
-                if not keep_cases:
-                    r.case = None
-                active[r.region] = r
-                queue.pop(0)
-            if not active:
-                if pending:
-                    time.sleep(max(0.0, pending[0].arrival - now()))
-                continue
-            b.run_graph(chunk, 1)
-            # while the device steps: generate the inputs of the next queued requests
-            arrivals()
+            # generation of the queued requests' inputs, oldest first, at most `lookahead` at a time
             for r in queue[:lookahead]:
-                build(r)
-            b.sync()
-            bond_steps += chunk * sum(r.nBonds for r in active.values())
-            st = b.status()
-            for reg, r in list(active.items()):
-                if st[reg]["finished"]:
-                    r.finished = now()
-                    r.steps = st[reg]["step"]
-                    r.peak = st[reg]["peak"]
-                    if keep_history:
-                        r.history = b.history(reg)
-                    b.retire(reg)
-                    del active[reg]
-                    done.append(r)
-                    if log:
-                        log(f"    request {r.rid} ({r.name}) done: {r.steps} steps, "
-                            f"{r.time_to_solution:.2f} s")
+                if r.case is None and r.rid not in futures:
+                    futures[r.rid] = pool.submit(build, r)
+
+        def collect(r, wait):
+            """The request's inputs, if generated (waiting for them if `wait`)."""
+            if r.case is not None:
+                return True
+            f = futures.get(r.rid)
+            if f is None or (not wait and not f.done()):
+                return False
+            r.case, dt = f.result()
+            del futures[r.rid]
+            r.built = now()
+            build_s.append(dt)
+            return True
+
+        try:
+            while pending or queue or active:
+                arrivals()
+                # admission, first come first served: the head waits for its inputs only when the
+                # device has nothing to run; a request that does not fit waits for slots
+                while queue:
+                    r = queue[0]
+                    tw = time.perf_counter()
+                    ok = collect(r, wait=not active)
+                    t_wait += time.perf_counter() - tw
+                    if not ok:
+                        break
+                    ta = time.perf_counter()
+                    try:
+                        r.region = b.admit(r.case, stop_fraction, stop_displacement)
+                    except BatchFull:
+                        break
+                    except BatchTooSmall as e:
+                        r.rejected = str(e)
+                        rejected.append(queue.pop(0))
+                        continue
+                    admit_s.append(time.perf_counter() - ta)
+                    r.admitted = now()
+                    r.name, r.nBonds = r.case.name, r.case.nBonds
+                    if not keep_cases:
+                        r.case = None
+                    active[r.region] = r
+                    queue.pop(0)
+                    arrivals()
+                if not active:
+                    if pending and not queue:
+                        time.sleep(max(0.0, pending[0].arrival - now()))
+                    continue
+                td = time.perf_counter()
+                b.run_graph(chunk, 1)
+                t_launch += time.perf_counter() - td
+                arrivals()
+                td = time.perf_counter()
+                b.sync()
+                t_sync += time.perf_counter() - td
+                bond_steps += chunk * sum(r.nBonds for r in active.values())
+                st = b.status()
+                for reg, r in list(active.items()):
+                    if st[reg]["finished"]:
+                        r.finished = now()
+                        r.steps = st[reg]["step"]
+                        r.peak = st[reg]["peak"]
+                        if keep_history:
+                            r.history = b.history(reg)
+                        b.retire(reg)
+                        del active[reg]
+                        done.append(r)
+                        if log:
+                            log(f"    request {r.rid} ({r.name}) done: {r.steps} steps, "
+                                f"{r.time_to_solution:.2f} s")
+        finally:
+            pool.shutdown(wait=True, cancel_futures=True)
         wall = now()
         torch.cuda.synchronize()
     tts = np.array([r.time_to_solution for r in done])
-    build_s = np.array([r.built - r.arrival for r in done if not math.isnan(r.built)])
     wait_s = np.array([r.admitted - r.arrival for r in done])
     return dict(requests=len(done), rejected=len(rejected), wall_s=wall, requests_per_s=len(done) / wall,
                 bond_updates_per_s=bond_steps / wall, p50_s=float(np.percentile(tts, 50)),
                 p90_s=float(np.percentile(tts, 90)), p99_s=float(np.percentile(tts, 99)), max_s=float(tts.max()),
-                build_after_arrival_s_mean=float(build_s.mean()) if build_s.size else None,
-                admission_wait_s_mean=float(wait_s.mean()), slots=slots, done=done)
+                build_s_mean=float(np.mean(build_s)) if build_s else None,
+                admit_s_mean=float(np.mean(admit_s)) if admit_s else None,
+                admission_wait_s_mean=float(wait_s.mean()), slots=slots,
+                wall_split=dict(launch_s=t_launch, device_wait_s=t_sync, admission_s=float(np.sum(admit_s)),
+                                input_wait_s=t_wait),
+                done=done)
