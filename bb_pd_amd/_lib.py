@@ -130,6 +130,9 @@ SIGNATURES = {
     "bbpd_batch_strain_energy": (_I, [_V, _I, _P, C.POINTER(_D)]),
     "bbpd_batch_create": (_I, [_P, _I, _I, C.POINTER(_V)]),
     "bbpd_batch_admit": (_I, [_V, _P, _D, _D, C.POINTER(_I)]),
+    "bbpd_batch_prepare": (_I, [_V, _P, C.POINTER(_V)]),
+    "bbpd_batch_admit_prepared": (_I, [_V, _V, _D, _D, C.POINTER(_I)]),
+    "bbpd_admission_free": (None, [_V]),
     "bbpd_batch_retire": (_I, [_V, _I]),
     "bbpd_batch_status": (_I, [_V, _P, _I]),
     "bbpd_batch_observables": (_I, [_V, _I, _P]),

@@ -24,6 +24,27 @@ class BatchTooSmall(_lib.BBPDError):
     """The problem needs more node slots than the whole batch holds: it can never be admitted."""
 
 
+class Admission:
+    """A problem checked and laid out for a batch on the host (bbpd_batch_prepare): admitting it then
+    only finds a region and slots and uploads.  Holds copies of what it needs: the case's arrays may be
+    dropped once it exists."""
+
+    def __init__(self, batch: "Batch", handle, nNodes: int, nBonds: int, name: str):
+        self._batch, self._h = batch, handle
+        self.nNodes, self.nBonds, self.name = nNodes, nBonds, name
+
+    def free(self):
+        if getattr(self, "_h", None):
+            self._batch._L.bbpd_admission_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Batch:
     def __init__(self, shape: BeamCase, precision: int = _lib.MIXED, max_regions: int = 64,
                  max_slots: int = 4 << 20, history_rows: int = 65536, device: int = 0):
@@ -40,6 +61,7 @@ class Batch:
         _lib.check(L.bbpd_batch_create(C.byref(spec), precision, device, C.byref(h)), "bbpd_batch_create")
         self._h = h
         self.cases: dict[int, BeamCase] = {}
+        self.sizes: dict[int, tuple] = {}  # region -> (nNodes, nBonds)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -59,23 +81,46 @@ class Batch:
         self.close()
 
     # ---- membership
-    def admit(self, case: BeamCase, stop_fraction: float = 0.0, stop_displacement: float = 0.0) -> int:
-        keep: list = []
-        prob = to_problem(case, keep)
-        r = C.c_int(-1)
-        rc = self._L.bbpd_batch_admit(self._h, C.byref(prob), float(stop_fraction), float(stop_displacement),
-                                      C.byref(r))
+    def _admit_rc(self, rc, what):
         if rc == _lib.ERR_FULL:
             raise BatchFull(self._L.bbpd_last_error().decode())
         if rc == _lib.ERR_NOMEM:
             raise BatchTooSmall(self._L.bbpd_last_error().decode())
-        _lib.check(rc, "bbpd_batch_admit")
+        _lib.check(rc, what)
+
+    def admit(self, case, stop_fraction: float = 0.0, stop_displacement: float = 0.0) -> int:
+        """Admit a case (bbpd_batch_admit), or an Admission prepared for this batch
+        (bbpd_batch_admit_prepared; freed once admitted)."""
+        r = C.c_int(-1)
+        if isinstance(case, Admission):
+            rc = self._L.bbpd_batch_admit_prepared(self._h, case._h, float(stop_fraction), float(stop_displacement),
+                                                   C.byref(r))
+            self._admit_rc(rc, "bbpd_batch_admit_prepared")
+            self.sizes[r.value] = (case.nNodes, case.nBonds)
+            case.free()
+            return r.value
+        keep: list = []
+        prob = to_problem(case, keep)
+        rc = self._L.bbpd_batch_admit(self._h, C.byref(prob), float(stop_fraction), float(stop_displacement),
+                                      C.byref(r))
+        self._admit_rc(rc, "bbpd_batch_admit")
         self.cases[r.value] = case
+        self.sizes[r.value] = (case.nNodes, case.nBonds)
         return r.value
+
+    def prepare(self, case: BeamCase) -> Admission:
+        """Check and lay out a case for this batch on the host only (bbpd_batch_prepare): may run on
+        another thread while the batch steps.  Raises BatchTooSmall if it can never fit."""
+        keep: list = []
+        prob = to_problem(case, keep)
+        h = C.c_void_p()
+        self._admit_rc(self._L.bbpd_batch_prepare(self._h, C.byref(prob), C.byref(h)), "bbpd_batch_prepare")
+        return Admission(self, h, case.nNodes, case.nBonds, case.name)
 
     def retire(self, region: int):
         _lib.check(self._L.bbpd_batch_retire(self._h, region), "bbpd_batch_retire")
         self.cases.pop(region, None)
+        self.sizes.pop(region, None)
 
     # ---- running (every active region advances one step per launch)
     def step(self, n: int = 1):
@@ -115,14 +160,14 @@ class Batch:
         return _lib.history_dict(r)
 
     def nodal(self, region: int):
-        nN = self.cases[region].nNodes
+        nN = self.sizes[region][0]
         arrs = [np.zeros((nN, 3), order="F") for _ in range(4)]
         p = [a.ctypes.data_as(C.c_void_p) for a in arrs]
         _lib.check(self._L.bbpd_batch_get_nodal(self._h, region, *p), "bbpd_batch_get_nodal")
         return tuple(arrs)
 
     def bonds(self, region: int):
-        nB = self.cases[region].nBonds
+        nB = self.sizes[region][1]
         arrs = [np.zeros(nB) for _ in range(3)]
         p = [a.ctypes.data_as(C.c_void_p) for a in arrs]
         _lib.check(self._L.bbpd_batch_get_bonds(self._h, region, *p), "bbpd_batch_get_bonds")
@@ -138,7 +183,7 @@ class Batch:
 
     def damage(self, region: int, energy: bool = True):
         """(damage field of the region's problem, total damage energy or None)."""
-        d = np.zeros(self.cases[region].nNodes)
+        d = np.zeros(self.sizes[region][0])
         e = C.c_double()
         _lib.check(self._L.bbpd_batch_damage(self._h, region, d.ctypes.data_as(C.c_void_p),
                                              C.byref(e) if energy else None), "bbpd_batch_damage")
@@ -146,7 +191,7 @@ class Batch:
 
     def strain_energy(self, region: int):
         """(strain energy density per node of the region's problem, total strain energy)."""
-        d = np.zeros(self.cases[region].nNodes)
+        d = np.zeros(self.sizes[region][0])
         e = C.c_double()
         _lib.check(self._L.bbpd_batch_strain_energy(self._h, region, d.ctypes.data_as(C.c_void_p), C.byref(e)),
                    "bbpd_batch_strain_energy")

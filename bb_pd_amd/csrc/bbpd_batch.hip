@@ -1,5 +1,7 @@
 // bbpd_batch.hip -- batches (bbpd_batch_*): many problem instances as regions of one set of
 // device arrays, stepped by one launch of each kernel.
+#include <memory>
+
 #include "bbpd_solver_impl.hpp"
 
 // ================================================================================================
@@ -250,11 +252,74 @@ extern "C" int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int
     return BBPD_OK;
 }
 
-extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double stop_frac, double stop_disp,
-                                int *region)
+// A problem checked and laid out for a batch on the host, in its own local slot and entry order: the
+// admission proper (bbpd_batch_admit_prepared) only finds it a region and a slot range, offsets
+// the local indices and uploads.
+struct bbpd_admission {
+    const bbpd_solver *batch = nullptr; // the batch it was prepared for
+    int nN = 0, nB = 0, need = 0;       // nodes, bonds, blocks of BLOCK slots
+    int noff = 0;
+    std::vector<int> slot;              // node -> local slot
+    std::vector<long long> ei, ej;      // bond -> local entry at its two ends
+    std::vector<NodeRec> hpos;
+    std::vector<double> hX, hrho, hm;
+    std::vector<unsigned char> hc;
+    std::vector<uint4> hmask;
+    std::vector<unsigned long long> hmt; // lane masks per (slice, offset)
+    bool any_failed = false;            // some bond starts failed: history and words uploaded
+    std::vector<unsigned char> h0, hs0, hsc; // entries in the device width (FP32 mixed, FP64)
+    std::vector<unsigned long long> hw; // history words (any_failed)
+    std::vector<int4> off;
+    std::vector<int2> rows;
+    std::vector<StOff> hst;
+    std::vector<double> tab;
+    std::vector<float> tabf;
+    std::vector<int> fam;               // members of the bodies in order, local slots
+    std::vector<Body> hb;               // bodies (fam pointers set at admission)
+    std::vector<int> nfam;
+    Region RG{};                        // the descriptor, less what depends on the region and slots
+    int ref = -1, cmod1 = -1, cmod2 = -1; // local observation slots
+};
+
+namespace bbpd_sv {
+
+// The room for a problem of nN nodes: BBPD_ERR_NOMEM when it needs more node slots than the batch
+// holds (it can never be admitted), BBPD_ERR_FULL when no region or slot range is free now.
+int batch_room(bbpd_solver *s, int nN, int *rid, int *blk0)
 {
-    BBPD_REQUIRE(s && s->batch && pr && region, "bad arguments");
-    *region = -1;
+    if ((long long)(nN + BLOCK - 1) / BLOCK > (long long)s->freeblk.size()) {
+        bbpd::set_error("the problem needs " + std::to_string(((long long)nN + BLOCK - 1) / BLOCK * BLOCK) +
+                        " node slots, more than the batch holds (" +
+                        std::to_string((long long)s->freeblk.size() * BLOCK) + ")");
+        return BBPD_ERR_NOMEM;
+    }
+    *rid = -1;
+    for (int r = 0; r < s->maxRegions && *rid < 0; r++)
+        if (!s->active[r]) *rid = r;
+    const int need = (nN + BLOCK - 1) / BLOCK;
+    *blk0 = -1;
+    const int nblk = (int)s->freeblk.size();
+    int run = 0;
+    for (int b = 0; b < nblk; b++) {
+        run = s->freeblk[b] ? run + 1 : 0;
+        if (run == need) {
+            *blk0 = b - need + 1;
+            break;
+        }
+    }
+    if (*rid < 0 || *blk0 < 0) {
+        bbpd::set_error(*rid < 0 ? "batch full: no free region" : "batch full: no free slot range");
+        return BBPD_ERR_FULL;
+    }
+    return BBPD_OK;
+}
+
+} // namespace bbpd_sv
+
+extern "C" int bbpd_batch_prepare(bbpd_solver *s, const bbpd_problem *pr, bbpd_admission **out)
+{
+    BBPD_REQUIRE(s && s->batch && pr && out, "bad arguments");
+    *out = nullptr;
     const bbpd_problem &sh = s->shape;
     const bool mixed = s->precision == BBPD_PRECISION_MIXED;
     const int nN = pr->nNodes, nB = pr->nBonds;
@@ -279,38 +344,10 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     BBPD_REQUIRE(pr->schedule != 2 || (pr->integrator == BBPD_INTEGRATOR_ADR && pr->eqTol >= 0 &&
                                        pr->itersPerIncrementMin >= 0),
                  "the equilibrium staircase needs ADR, eqTol >= 0 and itersPerIncrementMin >= 0");
-    // room first (a region and a free slot range), before the set-up checks and the plan: a caller
-    // that retries while the batch is full pays only this test
-    // a problem larger than the whole slot pool can never be admitted: a distinct error, so that a
-    // caller does not wait for slots that never free up
     if ((long long)(nN + BLOCK - 1) / BLOCK > (long long)s->freeblk.size()) {
-        bbpd::set_error("the problem needs " + std::to_string(((long long)nN + BLOCK - 1) / BLOCK * BLOCK) +
-                        " node slots, more than the batch holds (" +
-                        std::to_string((long long)s->freeblk.size() * BLOCK) + ")");
-        return BBPD_ERR_NOMEM;
+        int rid, blk0;
+        return batch_room(s, nN, &rid, &blk0); // the never-fits error
     }
-    // region and slot range
-    int rid = -1;
-    for (int r = 0; r < s->maxRegions && rid < 0; r++)
-        if (!s->active[r]) rid = r;
-    const int need = (nN + BLOCK - 1) / BLOCK;
-    int blk0 = -1;
-    {
-        const int nblk = (int)s->freeblk.size();
-        int run = 0;
-        for (int b = 0; b < nblk; b++) {
-            run = s->freeblk[b] ? run + 1 : 0;
-            if (run == need) {
-                blk0 = b - need + 1;
-                break;
-            }
-        }
-    }
-    if (rid < 0 || blk0 < 0) {
-        bbpd::set_error(rid < 0 ? "batch full: no free region" : "batch full: no free slot range");
-        return BBPD_ERR_FULL;
-    }
-    BBPD_HIP(hipSetDevice(s->device));
     int rc;
     std::vector<int> bi, bj;
     if ((rc = check_bonds(pr, bi, bj))) return rc;
@@ -329,72 +366,88 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     int famtot = 0;
     for (int b = 0; b < pr->nBodies; b++) famtot += pr->nFamily[b];
     BBPD_REQUIRE(famtot <= RB_FAM, "contact families too large for a batch region");
-    const int base = blk0 * BLOCK, nSl = need * BLOCK;
-    const size_t nS = (size_t)s->nSlots;
-    const long long e0 = (long long)base * ST_MAX;
-    const int noff = sp.noff;
-    BBPD_HIP(hipStreamSynchronize(s->stream));
-    long long tick = 0;
-    BBPD_HIP(hipMemcpy(&tick, s->d_tick, sizeof(tick), hipMemcpyDeviceToHost));
 
-    // node data of the region's slots
-    std::vector<NodeRec> hpos(nSl);
-    std::memset(hpos.data(), 0, sizeof(NodeRec) * nSl);
-    std::vector<double> hX(3 * (size_t)nSl, 0.0), hrho(nSl, 1.0), hm(nSl, 1.0);
-    std::vector<unsigned char> hc(nSl, 0);
-    std::vector<uint4> hmask(nSl, make_uint4(0, 0, 0, 0));
-    auto &RH = s->rh[rid];
-    RH.nN = nN;
-    RH.nB = nB;
-    RH.slot_of.assign(nN, 0);
+    std::unique_ptr<bbpd_admission> A(new bbpd_admission());
+    A->batch = s;
+    A->nN = nN;
+    A->nB = nB;
+    A->need = (nN + BLOCK - 1) / BLOCK;
+    const int nSl = A->need * BLOCK;
+    const int noff = sp.noff;
+    A->noff = noff;
+    A->slot = sp.slot;
+
+    // node data in local slot order
+    A->hpos.assign(nSl, NodeRec{});
+    std::memset(A->hpos.data(), 0, sizeof(NodeRec) * nSl);
+    A->hX.assign(3 * (size_t)nSl, 0.0);
+    A->hrho.assign(nSl, 1.0);
+    A->hm.assign(nSl, 1.0);
+    A->hc.assign(nSl, 0);
+    A->hmask.assign(nSl, make_uint4(0, 0, 0, 0));
     for (int n = 0; n < nN; n++) {
-        const int q = sp.slot[n]; // local slot
-        RH.slot_of[n] = base + q;
+        const int q = sp.slot[n];
         double xv[3];
         for (int d = 0; d < 3; d++) {
             xv[d] = pr->undeformedCoordinates[n + (size_t)nN * d];
-            hX[q + (size_t)nSl * d] = xv[d];
-            if (pr->CONSTRAINTFLAG && pr->CONSTRAINTFLAG[n + (size_t)nN * d] == 1) hc[q] |= (unsigned char)(1u << d);
+            A->hX[q + (size_t)nSl * d] = xv[d];
+            if (pr->CONSTRAINTFLAG && pr->CONSTRAINTFLAG[n + (size_t)nN * d] == 1) A->hc[q] |= (unsigned char)(1u << d);
         }
-        hpos[q].x = mixed ? 0.0 : xv[0];
-        hpos[q].y = mixed ? 0.0 : xv[1];
-        hpos[q].z = mixed ? 0.0 : xv[2];
-        hpos[q].q = (uint64_t)sp.k[3 * (size_t)n] | ((uint64_t)sp.k[3 * (size_t)n + 1] << LAT_BITS) |
-                    ((uint64_t)sp.k[3 * (size_t)n + 2] << (2 * LAT_BITS)) | ((uint64_t)sp.gidx[n] << G_SHIFT);
-        hrho[q] = pr->DENSITY[n];
-        if (pr->massVector) hm[q] = pr->massVector[n];
+        A->hpos[q].x = mixed ? 0.0 : xv[0];
+        A->hpos[q].y = mixed ? 0.0 : xv[1];
+        A->hpos[q].z = mixed ? 0.0 : xv[2];
+        A->hpos[q].q = (uint64_t)sp.k[3 * (size_t)n] | ((uint64_t)sp.k[3 * (size_t)n + 1] << LAT_BITS) |
+                       ((uint64_t)sp.k[3 * (size_t)n + 2] << (2 * LAT_BITS)) | ((uint64_t)sp.gidx[n] << G_SHIFT);
+        A->hrho[q] = pr->DENSITY[n];
+        if (pr->massVector) A->hm[q] = pr->massVector[n];
     }
     // entries: masks, history (sign bit set for bonds that start failed; all +0 when none does, which
-    // is set on the device), streamed parameters
+    // is set on the device), streamed parameters, in the device width
     const long long nEr = (long long)nSl * noff;
-    const bool any_failed = pr->fail && std::any_of(pr->fail, pr->fail + nB, [](double f) { return f == 0; });
-    std::vector<double> h0(any_failed ? nEr : 0, 0.0), hs0(pr->s0 ? nEr : 0, 0.0), hsc(pr->sc ? nEr : 0, 0.0);
-    RH.entry_i.assign(nB, 0);
-    RH.entry_j.assign(nB, 0);
+    const size_t w = mixed ? 4 : 8;
+    A->any_failed = pr->fail && std::any_of(pr->fail, pr->fail + nB, [](double f) { return f == 0; });
+    std::vector<double> h0(A->any_failed ? nEr : 0, 0.0), hs0(pr->s0 ? nEr : 0, 0.0), hsc(pr->sc ? nEr : 0, 0.0);
+    A->ei.assign(nB, 0);
+    A->ej.assign(nB, 0);
     auto dense_local = [&](int q, int o) { return ((long long)(q / SLICE) * noff + o) * SLICE + (q % SLICE); };
     for (int k = 0; k < nB; k++) {
         const int qi = sp.slot[bi[k]], qj = sp.slot[bj[k]];
         const long long ei = dense_local(qi, sp.oi[k]), ej = dense_local(qj, sp.oj[k]);
-        RH.entry_i[k] = e0 + ei;
-        RH.entry_j[k] = e0 + ej;
-        reinterpret_cast<uint32_t *>(&hmask[qi])[sp.oi[k] >> 5] |= 1u << (sp.oi[k] & 31);
-        reinterpret_cast<uint32_t *>(&hmask[qj])[sp.oj[k] >> 5] |= 1u << (sp.oj[k] & 31);
-        if (any_failed && pr->fail[k] == 0) h0[ei] = h0[ej] = -0.0;
+        A->ei[k] = ei;
+        A->ej[k] = ej;
+        reinterpret_cast<uint32_t *>(&A->hmask[qi])[sp.oi[k] >> 5] |= 1u << (sp.oi[k] & 31);
+        reinterpret_cast<uint32_t *>(&A->hmask[qj])[sp.oj[k] >> 5] |= 1u << (sp.oj[k] & 31);
+        if (A->any_failed && pr->fail[k] == 0) h0[ei] = h0[ej] = -0.0;
         if (pr->s0) hs0[ei] = hs0[ej] = pr->s0[k];
         if (pr->sc) hsc[ei] = hsc[ej] = pr->sc[k];
     }
-    // contact bodies (families as absolute slots), contact-member bits
-    std::vector<Body> hb(BBPD_MAX_BODIES);
-    std::vector<int> fam;
+    A->hmt = lane_masks(A->hmask, nSl, noff);
+    auto to_width = [&](const std::vector<double> &v, std::vector<unsigned char> &dst) {
+        dst.resize(v.size() * w);
+        if (mixed) {
+            float *f = reinterpret_cast<float *>(dst.data());
+            for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
+        } else {
+            std::memcpy(dst.data(), v.data(), v.size() * sizeof(double));
+        }
+    };
+    if (A->any_failed) {
+        A->hw = history_words(h0.data(), nSl / SLICE, noff);
+        to_width(h0, A->h0);
+    }
+    if (pr->s0) to_width(hs0, A->hs0);
+    if (pr->sc) to_width(hsc, A->hsc);
+    // contact bodies (families as local slots), contact-member bits
+    A->hb.assign(BBPD_MAX_BODIES, Body{});
+    A->nfam.assign(BBPD_MAX_BODIES, 0);
     std::vector<unsigned char> in_family(nN, 0);
     std::vector<int> owner(nN, -1);
     int seq_contact = 0; // families that share members are applied in order by one block
-    int *fambase = s->d_fampool + (size_t)rid * RB_FAM;
     for (int bdy = 0; bdy < pr->nBodies; bdy++) {
         const int nf = pr->nFamily[bdy];
         BBPD_REQUIRE(nf >= 0 && (nf == 0 || pr->family[bdy]), "contact family missing");
-        Body &B = hb[bdy];
-        B.fam = fambase + fam.size();
+        Body &B = A->hb[bdy];
+        A->nfam[bdy] = nf;
         B.nFam = nf;
         for (int t = 0; t < nf; t++) {
             const double idd = pr->family[bdy][t];
@@ -410,50 +463,43 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
             if (owner[id - 1] >= 0) seq_contact = 1;
             owner[id - 1] = bdy;
             in_family[id - 1] = 1;
-            fam.push_back(base + sp.slot[id - 1]);
+            A->fam.push_back(sp.slot[id - 1]);
         }
         B.cx = pr->centreX[bdy];
         B.cz = pr->centreZ[bdy];
         B.R = pr->radius[bdy];
         B.moves = pr->moves[bdy];
-        B.region = rid;
         B.index = bdy;
     }
     if (pr->integrator == BBPD_INTEGRATOR_ADR)
         for (int n = 0; n < nN; n++)
-            if (in_family[n]) hc[sp.slot[n]] |= CONTACT_BIT;
+            if (in_family[n]) A->hc[sp.slot[n]] |= CONTACT_BIT;
     // per-offset data and key tables
     const double hmr = sh.horizon - sh.RADIJ, hpr = sh.horizon + sh.RADIJ, two_r = 2 * sh.RADIJ;
     const double hdx[3] = {s->DX, s->DX, s->DX};
     int fast64 = 0;
-    const std::vector<StOff> hst =
-        offset_records(sp.off, hdx, hmr, hpr, sh.horizon, two_r, mixed, sp.kmin, sp.kmax, &fast64);
-    std::vector<double> tab(sp.tc);
-    tab.insert(tab.end(), sp.ts0.begin(), sp.ts0.end());
-    tab.insert(tab.end(), sp.tsc.begin(), sp.tsc.end());
-    BBPD_REQUIRE((int)tab.size() <= RB_TAB, "key tables too large for a batch region");
-    std::vector<float> tabf(tab.size());
-    for (size_t i = 0; i < sp.tc.size(); i++) tabf[i] = (float)(sp.tc[i] * sh.cellVolume);
-    for (size_t i = sp.tc.size(); i < tab.size(); i++) tabf[i] = (float)tab[i];
+    A->hst = offset_records(sp.off, hdx, hmr, hpr, sh.horizon, two_r, mixed, sp.kmin, sp.kmax, &fast64);
+    A->off = sp.off;
+    A->rows = sp.rows;
+    A->tab = sp.tc;
+    A->tab.insert(A->tab.end(), sp.ts0.begin(), sp.ts0.end());
+    A->tab.insert(A->tab.end(), sp.tsc.begin(), sp.tsc.end());
+    BBPD_REQUIRE((int)A->tab.size() <= RB_TAB, "key tables too large for a batch region");
+    A->tabf.resize(A->tab.size());
+    for (size_t i = 0; i < sp.tc.size(); i++) A->tabf[i] = (float)(sp.tc[i] * sh.cellVolume);
+    for (size_t i = sp.tc.size(); i < A->tab.size(); i++) A->tabf[i] = (float)A->tab[i];
 
-    // descriptor
-    Region RG{};
-    RG.base = base;
+    // descriptor, less what depends on the region and its slots
+    Region &RG = A->RG;
     RG.nN = nN;
     RG.nSlots = nSl;
-    RG.blk0 = blk0;
-    RG.nblk = need;
-    RG.e0 = e0;
+    RG.nblk = A->need;
     RG.noff = noff;
-    RG.off0 = rid * ST_MAX;
     RG.nrow = sp.nrow;
-    RG.row0 = rid * (ST_MAX + ST_PAD);
     RG.rx = sp.rx;
-    RG.sst0 = rid * (ST_MAX + ST_PAD);
     RG.fast64 = fast64;
     RG.kg = sp.G;
-    RG.tab0 = rid * RB_TAB;
-    RG.ntab = (int)tab.size();
+    RG.ntab = (int)A->tab.size();
     RG.ts0 = (int)sp.tc.size();
     RG.tsc = (int)(sp.tc.size() + sp.ts0.size());
     // key-tabled s0 / sc are not used in a batch: per-bond values are streamed
@@ -462,7 +508,6 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.s1 = pr->s1;
     RG.s0lo = pr->s0 && nB > 0 ? *std::min_element(pr->s0, pr->s0 + nB) : 0.0;
     RG.sclo = pr->sc && nB > 0 ? *std::min_element(pr->sc, pr->sc + nB) : 0.0;
-    RG.body0 = rid * BBPD_MAX_BODIES;
     RG.nBodies = pr->nBodies;
     RG.seq_contact = seq_contact;
     RG.schedule = pr->schedule;
@@ -472,10 +517,67 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     RG.increment = pr->increment;
     RG.eq_tol = pr->eqTol;
     RG.ipi_min = pr->itersPerIncrementMin;
-    auto slot_or_none = [&](int id1) { return (id1 >= 1 && id1 <= nN) ? base + sp.slot[id1 - 1] : -1; };
-    RG.ref_slot = slot_or_none(pr->referenceNode);
-    RG.cmod1 = slot_or_none(pr->cmodNode1);
-    RG.cmod2 = slot_or_none(pr->cmodNode2);
+    auto local_or_none = [&](int id1) { return (id1 >= 1 && id1 <= nN) ? sp.slot[id1 - 1] : -1; };
+    A->ref = local_or_none(pr->referenceNode);
+    A->cmod1 = local_or_none(pr->cmodNode1);
+    A->cmod2 = local_or_none(pr->cmodNode2);
+    *out = A.release();
+    return BBPD_OK;
+}
+
+extern "C" void bbpd_admission_free(bbpd_admission *a) { delete a; }
+
+extern "C" int bbpd_batch_admit_prepared(bbpd_solver *s, bbpd_admission *A, double stop_frac, double stop_disp,
+                                         int *region)
+{
+    BBPD_REQUIRE(s && s->batch && A && region, "bad arguments");
+    BBPD_REQUIRE(A->batch == s, "the admission was prepared for another batch");
+    *region = -1;
+    const bool mixed = s->precision == BBPD_PRECISION_MIXED;
+    int rid, blk0, rc;
+    if ((rc = batch_room(s, A->nN, &rid, &blk0))) return rc;
+    BBPD_HIP(hipSetDevice(s->device));
+    const int need = A->need, nSl = need * BLOCK, noff = A->noff;
+    const int base = blk0 * BLOCK;
+    const long long e0 = (long long)base * ST_MAX;
+    const long long nEr = (long long)nSl * noff;
+    BBPD_HIP(hipStreamSynchronize(s->stream));
+
+    auto &RH = s->rh[rid];
+    RH.nN = A->nN;
+    RH.nB = A->nB;
+    RH.slot_of.resize(A->nN);
+    for (int n = 0; n < A->nN; n++) RH.slot_of[n] = base + A->slot[n];
+    RH.entry_i.resize(A->nB);
+    RH.entry_j.resize(A->nB);
+    for (int k = 0; k < A->nB; k++) {
+        RH.entry_i[k] = e0 + A->ei[k];
+        RH.entry_j[k] = e0 + A->ej[k];
+    }
+    int *fambase = s->d_fampool + (size_t)rid * RB_FAM;
+    std::vector<int> fam(A->fam.size());
+    for (size_t t = 0; t < fam.size(); t++) fam[t] = base + A->fam[t];
+    std::vector<Body> hb(A->hb);
+    {
+        size_t f0 = 0;
+        for (int bdy = 0; bdy < A->RG.nBodies; bdy++) {
+            hb[bdy].fam = fambase + f0;
+            hb[bdy].region = rid;
+            f0 += (size_t)A->nfam[bdy];
+        }
+    }
+    Region RG = A->RG;
+    RG.base = base;
+    RG.blk0 = blk0;
+    RG.e0 = e0;
+    RG.off0 = rid * ST_MAX;
+    RG.row0 = rid * (ST_MAX + ST_PAD);
+    RG.sst0 = rid * (ST_MAX + ST_PAD);
+    RG.tab0 = rid * RB_TAB;
+    RG.body0 = rid * BBPD_MAX_BODIES;
+    RG.ref_slot = A->ref >= 0 ? base + A->ref : -1;
+    RG.cmod1 = A->cmod1 >= 0 ? base + A->cmod1 : -1;
+    RG.cmod2 = A->cmod2 >= 0 ? base + A->cmod2 : -1;
     RG.hist0 = (long long)rid * s->histRows;
     RG.hist_cap = s->histRows;
     RG.stop_frac = stop_frac;
@@ -483,58 +585,48 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
 
     // upload
     Params &P = s->P;
-    for (int q = 0; q < 2; q++) BBPD_HIP(hipMemcpy(P.pos[q] + base, hpos.data(), sizeof(NodeRec) * nSl, hipMemcpyHostToDevice));
+    for (int q = 0; q < 2; q++)
+        BBPD_HIP(hipMemcpy(P.pos[q] + base, A->hpos.data(), sizeof(NodeRec) * nSl, hipMemcpyHostToDevice));
     double *Xc[3] = {(double *)P.X0, (double *)P.X1, (double *)P.X2};
     double *uc[3] = {P.u0, P.u1, P.u2}, *vc[3] = {P.v0, P.v1, P.v2}, *fc[3] = {P.F0, P.F1, P.F2};
     double *fpc[3] = {P.Fp0, P.Fp1, P.Fp2};
     for (int d = 0; d < 3; d++) {
-        BBPD_HIP(hipMemcpy(Xc[d] + base, hX.data() + (size_t)nSl * d, sizeof(double) * nSl, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(Xc[d] + base, A->hX.data() + (size_t)nSl * d, sizeof(double) * nSl, hipMemcpyHostToDevice));
         // zero fills on the batch's stream (outside any captured graph), ordered before its next steps
         if (uc[d]) BBPD_HIP(hipMemsetAsync(uc[d] + base, 0, sizeof(double) * nSl, s->stream));
         BBPD_HIP(hipMemsetAsync(vc[d] + base, 0, sizeof(double) * nSl, s->stream));
         BBPD_HIP(hipMemsetAsync(fc[d] + base, 0, sizeof(double) * nSl, s->stream));
         if (fpc[d]) BBPD_HIP(hipMemsetAsync(fpc[d] + base, 0, sizeof(double) * nSl, s->stream));
     }
-    BBPD_HIP(hipMemcpy((double *)P.rho + base, hrho.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
-    if (P.mass) BBPD_HIP(hipMemcpy((double *)P.mass + base, hm.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy((unsigned char *)P.cons + base, hc.data(), nSl, hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy((uint4 *)P.smask + base, hmask.data(), sizeof(uint4) * nSl, hipMemcpyHostToDevice));
-    {
-        const std::vector<unsigned long long> hmt = lane_masks(hmask, nSl, noff);
-        BBPD_HIP(hipMemcpy((unsigned long long *)P.smaskT + RG.e0 / SLICE, hmt.data(),
-                           sizeof(unsigned long long) * hmt.size(), hipMemcpyHostToDevice));
-    }
-    auto put_entries = [&](void *dst, const std::vector<double> &v) -> int {
-        if (mixed) {
-            std::vector<float> f(v.size());
-            for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
-            BBPD_HIP(hipMemcpy((float *)dst + e0, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
-        } else {
-            BBPD_HIP(hipMemcpy((double *)dst + e0, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
-        }
-        return BBPD_OK;
-    };
-    if (any_failed) {
-        if ((rc = put_entries(P.hs, h0))) return rc;
-        const std::vector<unsigned long long> hw = history_words(h0.data(), nSl / SLICE, noff);
-        BBPD_HIP(hipMemcpy(P.snz + 2 * (base / SLICE), hw.data(), sizeof(unsigned long long) * hw.size(),
+    BBPD_HIP(hipMemcpy((double *)P.rho + base, A->hrho.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
+    if (P.mass) BBPD_HIP(hipMemcpy((double *)P.mass + base, A->hm.data(), sizeof(double) * nSl, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((unsigned char *)P.cons + base, A->hc.data(), nSl, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((uint4 *)P.smask + base, A->hmask.data(), sizeof(uint4) * nSl, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((unsigned long long *)P.smaskT + e0 / SLICE, A->hmt.data(),
+                       sizeof(unsigned long long) * A->hmt.size(), hipMemcpyHostToDevice));
+    const size_t w = mixed ? 4 : 8;
+    if (A->any_failed) {
+        BBPD_HIP(hipMemcpy((char *)P.hs + (size_t)e0 * w, A->h0.data(), A->h0.size(), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(P.snz + 2 * (base / SLICE), A->hw.data(), sizeof(unsigned long long) * A->hw.size(),
                            hipMemcpyHostToDevice));
     } else {
-        BBPD_HIP(hipMemsetAsync((char *)P.hs + (size_t)e0 * (mixed ? 4 : 8), 0, (size_t)nEr * (mixed ? 4 : 8),
-                                s->stream));
+        BBPD_HIP(hipMemsetAsync((char *)P.hs + (size_t)e0 * w, 0, (size_t)nEr * w, s->stream));
         BBPD_HIP(hipMemsetAsync(P.snz + 2 * (base / SLICE), 0, sizeof(unsigned long long) * 2 * (nSl / SLICE),
                                 s->stream));
     }
-    if (pr->s0 && (rc = put_entries((void *)P.s0str, hs0))) return rc;
-    if (pr->sc && (rc = put_entries((void *)P.scstr, hsc))) return rc;
-    BBPD_HIP(hipMemcpy((int4 *)P.soff + RG.off0, sp.off.data(), sizeof(int4) * noff, hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy((int2 *)P.srow + RG.row0, sp.rows.data(), sizeof(int2) * sp.rows.size(),
-                       hipMemcpyHostToDevice));
-    BBPD_HIP(hipMemcpy((StOff *)P.sst + RG.sst0, hst.data(), sizeof(StOff) * hst.size(), hipMemcpyHostToDevice));
-    if (!tab.empty()) {
-        BBPD_HIP(hipMemcpy((double *)P.tab + RG.tab0, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    if (!A->hs0.empty())
+        BBPD_HIP(hipMemcpy((char *)P.s0str + (size_t)e0 * w, A->hs0.data(), A->hs0.size(), hipMemcpyHostToDevice));
+    if (!A->hsc.empty())
+        BBPD_HIP(hipMemcpy((char *)P.scstr + (size_t)e0 * w, A->hsc.data(), A->hsc.size(), hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((int4 *)P.soff + RG.off0, A->off.data(), sizeof(int4) * noff, hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((int2 *)P.srow + RG.row0, A->rows.data(), sizeof(int2) * A->rows.size(), hipMemcpyHostToDevice));
+    BBPD_HIP(hipMemcpy((StOff *)P.sst + RG.sst0, A->hst.data(), sizeof(StOff) * A->hst.size(), hipMemcpyHostToDevice));
+    if (!A->tab.empty()) {
+        BBPD_HIP(hipMemcpy((double *)P.tab + RG.tab0, A->tab.data(), sizeof(double) * A->tab.size(),
+                           hipMemcpyHostToDevice));
         if (P.tabf)
-            BBPD_HIP(hipMemcpy((float *)P.tabf + RG.tab0, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice));
+            BBPD_HIP(hipMemcpy((float *)P.tabf + RG.tab0, A->tabf.data(), sizeof(float) * A->tabf.size(),
+                               hipMemcpyHostToDevice));
     }
     if (!fam.empty()) BBPD_HIP(hipMemcpy(fambase, fam.data(), sizeof(int) * fam.size(), hipMemcpyHostToDevice));
     BBPD_HIP(hipMemcpy(s->d_bodies + RG.body0, hb.data(), sizeof(Body) * BBPD_MAX_BODIES, hipMemcpyHostToDevice));
@@ -550,6 +642,23 @@ extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double s
     if ((rc = batch_rebuild(s))) return rc;
     *region = rid;
     return BBPD_OK;
+}
+
+extern "C" int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *pr, double stop_frac, double stop_disp,
+                                int *region)
+{
+    BBPD_REQUIRE(s && s->batch && pr && region, "bad arguments");
+    *region = -1;
+    // room first (a region and a free slot range), before the set-up checks and the plan: a caller
+    // that retries while the batch is full pays only this test
+    int rid, blk0, rc;
+    BBPD_REQUIRE(pr->nNodes > 0, "need nNodes > 0 and nBonds >= 0");
+    if ((rc = batch_room(s, pr->nNodes, &rid, &blk0))) return rc;
+    bbpd_admission *A = nullptr;
+    if ((rc = bbpd_batch_prepare(s, pr, &A))) return rc;
+    rc = bbpd_batch_admit_prepared(s, A, stop_frac, stop_disp, region);
+    bbpd_admission_free(A);
+    return rc;
 }
 
 extern "C" int bbpd_batch_retire(bbpd_solver *s, int region)

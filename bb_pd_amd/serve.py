@@ -83,15 +83,15 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
           max_regions: int = 64, keep_history: bool = False, keep_cases: bool = False,
           closed_loop: bool = False, build_workers: int = 8, lookahead: int = 16, log=None) -> dict:
     """Run the service over `requests` (their `arrival` times are honoured unless `closed_loop`,
-    in which case all are queued at t = 0).  A request that carries only its parameters has its
-    inputs generated after it arrives, by a pool of `build_workers` host threads (the lattice
-    generator releases the interpreter lock), for at most `lookahead` queued requests at a time, so
-    input generation overlaps the device's steps and other requests' generation, and counts in the
-    request's time to solution.  Requests are admitted first come, first served, once their inputs
-    exist and the batch has a region and the node slots; once admitted, a request's host arrays are
-    dropped (unless `keep_cases`) and its state lives on the device until it is retired.  A request
-    that needs more node slots than the whole batch holds is rejected.  Returns per-request times and
-    aggregate figures."""
+    in which case all are queued at t = 0).  Once a request has arrived, a pool of `build_workers`
+    host threads generates its inputs (if it carries only its parameters) and checks and lays them
+    out for the batch (`Batch.prepare`), for at most `lookahead` queued requests at a time: that work
+    overlaps the device's steps and the other requests', and counts in the request's time to
+    solution.  Requests are admitted first come, first served, once prepared and once the batch has
+    a region and the node slots; admission then only uploads.  A request's host arrays are dropped
+    once it is prepared (unless `keep_cases`): its state lives on the device until it is retired.  A
+    request that needs more node slots than the whole batch holds is rejected.  Returns per-request
+    times and aggregate figures."""
     import concurrent.futures as cf
     import torch
     slots = int(hbm_budget // slot_bytes(precision))
@@ -101,12 +101,14 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
     done: list = []
     rejected: list = []
     futures: dict = {}
+    ready: dict = {}  # rid -> prepared admission
     admit_s: list = []
     build_s: list = []
-    # wall time launching device chunks (a graph is captured again when the membership changed), waiting
-    # for them, and waiting for a request's inputs
-    t_launch = t_sync = t_wait = 0.0
+    prep_s: list = []
     bond_steps = 0
+    # wall time launching device chunks (a graph is captured again when the membership changed), waiting
+    # for them, admitting, and waiting for a request's inputs
+    t_launch = t_sync = t_wait = 0.0
     # the batch's shared scalars (law, integrator, time step, spacing, horizon) from a request of
     # the same workload, built before the clock starts
     shape = pending[0].case if pending[0].case is not None else build_request(pending[0].params, ipi, increment_um)
@@ -116,37 +118,47 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
 
-        def build(r):
+        def prepare(r):
+            """On a worker thread: the request's inputs, then its admission."""
             t = time.perf_counter()
-            case = build_request(r.params, ipi, increment_um)
-            return case, time.perf_counter() - t
+            case = r.case if r.case is not None else build_request(r.params, ipi, increment_um)
+            tb = time.perf_counter() - t
+            try:
+                adm, err = b.prepare(case), ""
+            except BatchTooSmall as e:
+                adm, err = None, str(e)
+            return case, adm, err, tb, time.perf_counter() - t - tb
 
         def arrivals():
             t = now()
             while pending and (closed_loop or pending[0].arrival <= t):
                 queue.append(pending.pop(0))
-            # generation of the queued requests' inputs, oldest first, at most `lookahead` at a time
+            # the queued requests' preparation, oldest first, at most `lookahead` at a time
             for r in queue[:lookahead]:
-                if r.case is None and r.rid not in futures:
-                    futures[r.rid] = pool.submit(build, r)
+                if r.rid not in futures and r.rid not in ready:
+                    futures[r.rid] = pool.submit(prepare, r)
 
         def collect(r, wait):
-            """The request's inputs, if generated (waiting for them if `wait`)."""
-            if r.case is not None:
+            """Whether the request is prepared (waiting for it if `wait`)."""
+            if r.rid in ready:
                 return True
             f = futures.get(r.rid)
             if f is None or (not wait and not f.done()):
                 return False
-            r.case, dt = f.result()
+            case, adm, err, tb, tp = f.result()
             del futures[r.rid]
             r.built = now()
-            build_s.append(dt)
+            r.name, r.nBonds = case.name, case.nBonds
+            r.case = case if keep_cases else None
+            build_s.append(tb)
+            prep_s.append(tp)
+            ready[r.rid] = (adm, err)
             return True
 
         try:
             while pending or queue or active:
                 arrivals()
-                # admission, first come first served: the head waits for its inputs only when the
+                # admission, first come first served: the head waits for its preparation only when the
                 # device has nothing to run; a request that does not fit waits for slots
                 while queue:
                     r = queue[0]
@@ -155,20 +167,20 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
                     t_wait += time.perf_counter() - tw
                     if not ok:
                         break
-                    ta = time.perf_counter()
-                    try:
-                        r.region = b.admit(r.case, stop_fraction, stop_displacement)
-                    except BatchFull:
-                        break
-                    except BatchTooSmall as e:
-                        r.rejected = str(e)
+                    adm, err = ready[r.rid]
+                    if adm is None:
+                        r.rejected = err
+                        del ready[r.rid]
                         rejected.append(queue.pop(0))
                         continue
+                    ta = time.perf_counter()
+                    try:
+                        r.region = b.admit(adm, stop_fraction, stop_displacement)
+                    except BatchFull:
+                        break
                     admit_s.append(time.perf_counter() - ta)
+                    del ready[r.rid]
                     r.admitted = now()
-                    r.name, r.nBonds = r.case.name, r.case.nBonds
-                    if not keep_cases:
-                        r.case = None
                     active[r.region] = r
                     queue.pop(0)
                     arrivals()
@@ -200,6 +212,9 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
                                 f"{r.time_to_solution:.2f} s")
         finally:
             pool.shutdown(wait=True, cancel_futures=True)
+            for adm, _ in ready.values():
+                if adm is not None:
+                    adm.free()
         wall = now()
         torch.cuda.synchronize()
     tts = np.array([r.time_to_solution for r in done])
@@ -208,6 +223,7 @@ def serve(requests: list, precision: int, hbm_budget: float, chunk: int, ipi: in
                 bond_updates_per_s=bond_steps / wall, p50_s=float(np.percentile(tts, 50)),
                 p90_s=float(np.percentile(tts, 90)), p99_s=float(np.percentile(tts, 99)), max_s=float(tts.max()),
                 build_s_mean=float(np.mean(build_s)) if build_s else None,
+                prepare_s_mean=float(np.mean(prep_s)) if prep_s else None,
                 admit_s_mean=float(np.mean(admit_s)) if admit_s else None,
                 admission_wait_s_mean=float(wait_s.mean()), slots=slots,
                 wall_split=dict(launch_s=t_launch, device_wait_s=t_sync, admission_s=float(np.sum(admit_s)),

@@ -605,7 +605,8 @@ def run_serve(n_requests: int, hbm_gb: float, ipi: int, increment_um: float, loa
                          requests_per_s=res["requests_per_s"], bond_updates_per_s=res["bond_updates_per_s"],
                          capacity_bond_updates_per_s=cap["bond_updates_per_s"], slots=res["slots"],
                          served=res["requests"], rejected=res["rejected"],
-                         build_s_mean=res["build_s_mean"], admit_s_mean=res["admit_s_mean"],
+                         build_s_mean=res["build_s_mean"], prepare_s_mean=res["prepare_s_mean"],
+                         admit_s_mean=res["admit_s_mean"],
                          admission_wait_s_mean=res["admission_wait_s_mean"], wall_s=res["wall_s"],
                          wall_split=res["wall_split"],
                          steps_min=int(min(steps)), steps_max=int(max(steps)))
@@ -698,7 +699,7 @@ def main():
                        "protocol": f"{args.serve_increment_um:g} um per {args.serve_ipi} ADR iterations",
                        "stop": "load < 20% of peak or 0.5 mm", "hbm_budget_gb": args.hbm_gb,
                        "time_to_solution": "arrival to retirement, input generation after arrival included",
-                       "input_generation": "8 host threads, at most 16 queued requests ahead"},
+                       "input_generation": "inputs generated and laid out for the batch (bbpd_batch_prepare) by 8 host threads, at most 16 queued requests ahead"},
             "mixed": m, "fp64": f,
         }), flush=True)
         return

@@ -367,6 +367,18 @@ int bbpd_batch_create(const bbpd_batch_spec *spec, int precision, int device, bb
  * node slots than the whole batch holds (it can never be admitted). */
 int bbpd_batch_admit(bbpd_solver *s, const bbpd_problem *problem, double stopFraction, double stopDisplacement,
                      int *region);
+/* Admission in two parts, so that a service can check and lay out arriving problems on host threads
+ * while the batch steps: bbpd_batch_prepare does everything bbpd_batch_admit does on the host (the
+ * checks, the stencil plan, the node and entry arrays in the problem's own slot order) and touches
+ * neither the device nor the batch's membership, so it may run on another host thread while the batch
+ * steps; bbpd_batch_admit_prepared then finds a region and a slot range and uploads.  Errors are those
+ * of bbpd_batch_admit; after BBPD_ERR_FULL the admission stays valid for a later call.  Free it with
+ * bbpd_admission_free once admitted or abandoned (not while a call that uses it runs). */
+typedef struct bbpd_admission bbpd_admission;
+int bbpd_batch_prepare(bbpd_solver *s, const bbpd_problem *problem, bbpd_admission **out);
+int bbpd_batch_admit_prepared(bbpd_solver *s, bbpd_admission *admission, double stopFraction,
+                              double stopDisplacement, int *region);
+void bbpd_admission_free(bbpd_admission *admission);
 int bbpd_batch_retire(bbpd_solver *s, int region);
 /* status of regions 0 .. n-1 (n <= maxRegions) in one device read */
 int bbpd_batch_status(bbpd_solver *s, bbpd_region_status *status, int n);

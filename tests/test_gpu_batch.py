@@ -175,3 +175,28 @@ def test_batch_rejects_mismatched_problems(gpu):
             setattr(shuffled, name, getattr(shuffled, name)[order])
         with pytest.raises(_lib.BBPDError, match="stencil layout"):
             b.admit(shuffled)
+
+
+def test_prepared_admission(gpu):
+    """bbpd_batch_prepare on another host thread while the batch steps, then bbpd_batch_admit_prepared:
+    the region runs as a directly admitted one (bit for bit against the single run); a prepared
+    admission refused for lack of room stays valid and is admitted once a region retires."""
+    import threading
+    first, second = ec(24, 6, 10), ec(48, 6, 10)   # 6 and 12 blocks of 256 slots: not both in 4096
+    print("case: prepared admission, 2 EC beams", flush=True)
+    ref = single(second, _lib.MIXED, 150)
+    with Batch(first, _lib.MIXED, max_regions=4, max_slots=4096) as b:
+        r0 = b.admit(first)
+        out = {}
+        t = threading.Thread(target=lambda: out.setdefault("adm", b.prepare(second)))
+        t.start()
+        b.step(100)                      # the batch steps while the other thread prepares
+        t.join()
+        b.sync()
+        adm = out["adm"]
+        with pytest.raises(BatchFull):
+            b.admit(adm)
+        b.retire(r0)
+        r1 = b.admit(adm)
+        b.step(150)
+        assert_region_equals(b, r1, ref)
