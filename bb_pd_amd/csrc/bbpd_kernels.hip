@@ -615,7 +615,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
                        : gb * BLOCK + (int)threadIdx.x;
     const int base = RG.base, noff = RG.noff, off0 = RG.off0, kg = RG.kg, tab0 = RG.tab0, ntab = RG.ntab;
     const int ts0 = RG.ts0, tsc = RG.tsc;
-    const bool fast64 = !MIXED && RG.fast64;
+    // a region's flag, made a scalar so that a test of it is a scalar branch
+    const bool fast64 = !MIXED && __builtin_amdgcn_readfirstlane(RG.fast64) != 0;
     const bool real = TILE ? wave_on : n - base < RG.nN;
     // the law's threshold: on the stretch (FP64 form), on x (mixed form, thr_to_x)
     auto thr_of = [](T t) -> T {
@@ -914,9 +915,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
             qy = fma(fma(-l, qy, dy), rl, qy);
             qz = fma(fma(-l, qz, dz), rl, qz);
             double gx = p * qx, gy = p * qy, gz = p * qz;
-            if (gx != gx) gx = 0.0;
-            if (gy != gy) gy = 0.0;
-            if (gz != gz) gz = 0.0;
+            // NaN -> 0 (calculatebondforcesMex.c:38) can only happen off the unscaled sequences: there
+            // every lane's length is at least 2^-383, |dX / l| <= 1, and s, the history, c, V and vcf
+            // are finite (a stencil offset is never zero, so L0 > 0)
+            if (!core) {
+                if (gx != gx) gx = 0.0;
+                if (gy != gy) gy = 0.0;
+                if (gz != gz) gz = 0.0;
+            }
             f0 += gx;
             f1 += gy;
             f2 += gz;
