@@ -588,7 +588,8 @@ __device__ __forceinline__ float thr_to_x(float t)
 // must allow (1 = its own choice), which bounds the registers per wave, trading instruction-level
 // for wave-level parallelism; REMAP keeps consecutive blocks on one XCD (its L2) or not.  PV says
 // where s0 and sc come from: PV_SCALAR the region's scalars, PV_TABLE both from key tables,
-// PV_GENERAL each from its scalar, key table or per-entry stream as the solver's modes say.
+// PV_GENERAL each from its scalar or per-entry stream, PV_GENTAB the same with a key table for one of
+// them, as the solver's modes say.
 template <bool MIXED, int LAW, int OCC, bool REMAP, int PV, bool TILE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && OCC > 1 ? 8 : OCC))) void k_bond_st(Params P)
 {
@@ -625,9 +626,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
     };
     const T rs0 = (T)RG.s0, rs1 = (T)RG.s1, rsc = (T)RG.sc, rthr = thr_of(law_threshold<LAW, T>(rs0, rsc, rs1));
     const T rs0lo = (T)RG.s0lo, rsclo = (T)RG.sclo;
-    // PV_GENERAL without key tables for s0 / sc: region scalars, a streamed value entering as the
-    // region's smallest (see below)
-    const bool gtab = P.s0mode == PM_TABLE || P.scmode == PM_TABLE;
+    // PV_GENERAL: region scalars, a streamed value entering as the region's smallest (see below)
+    constexpr bool GEN = PV == PV_GENERAL || PV == PV_GENTAB;
     const T gs0 = P.s0mode == PM_STREAM ? rs0lo : rs0, gsc = P.scmode == PM_STREAM ? rsclo : rsc;
     const T gthr = thr_of(law_threshold<LAW, T>(gs0, gsc, rs1));
     const T lk = (T)P.k, lalpha = (T)P.alpha, ldenom = (T)P.denom_exp, lsteel = (T)P.sc_steel, lff = (T)P.ff;
@@ -653,8 +653,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
     // a neighbour's staged key word: its family-size index, or (PV_SCALAR / PV_TABLE) that index's
     // byte offset in a row of the key table; the node's row `krow` is then an LDS byte address, so one
     // add gives the address of the entry's table record
-    constexpr int KSCALE = PV == PV_GENERAL ? 1 : (PV == PV_TABLE ? 2 : 1) * (int)sizeof(T);
-    const int tabl = PV == PV_GENERAL ? 0 : (int)lds_addr(tabs);
+    constexpr int KSCALE = GEN ? 1 : (PV == PV_TABLE ? 2 : 1) * (int)sizeof(T);
+    const int tabl = GEN ? 0 : (int)lds_addr(tabs);
 
     const NodeRec me = pos[n];
     const double qn0 = me.x, qn1 = me.y, qn2 = me.z;
@@ -670,8 +670,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
     const long long eslice = TILE ? RG.e0 : RG.e0 + ((long long)((gb * BLOCK - base) / SLICE) + wv) * noff * SLICE;
     const uint32_t sbytes = TILE ? (uint32_t)(((long long)RG.nSlots * noff + 2 * SLICE) * TW) : (uint32_t)noff * SLICE * TW;
     const brsrc rh = make_rsrc((const T *)P.hs + eslice, sbytes);
-    const brsrc r0 = PV == PV_GENERAL && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
-    const brsrc r1 = PV == PV_GENERAL && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
+    const brsrc r0 = GEN && P.s0mode == PM_STREAM ? make_rsrc((const T *)P.s0str + eslice, sbytes) : rh;
+    const brsrc r1 = GEN && P.scmode == PM_STREAM ? make_rsrc((const T *)P.scstr + eslice, sbytes) : rh;
     const uint32_t lb = TILE ? (uint32_t)(((n - base) / SLICE * noff * SLICE + (n - base) % SLICE) * (int)TW)
                              : (uint32_t)lane * TW;
     // Neighbour records, staged per stencil row (the offsets that share dy and dz, consecutive in
@@ -794,7 +794,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
     int cyz = INT_MIN;
     double eyy = 0, ezz = 0;
     auto compute = [&](const StEnt<T> &d, int o, int k, bool bit, float3 &a) {
-        if (!MIXED && PV != PV_GENERAL && d.st.dyz != cyz) {
+        if (!MIXED && !GEN && d.st.dyz != cyz) {
             cyz = d.st.dyz;
             const double ey = Xn1 - P.hy * (double)(ky + d.st.dy()), ez = Xn2 - P.hz * (double)(kz + d.st.dz());
             eyy = ey * ey;
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
         const double nbx = nb[0], nby = nb[RS], nbz = nb[2 * RS];
         const int key = krow + *reinterpret_cast<const int *>(nb + 3 * RS);
         // the entry's key-table values: c (c * V in the mixed form), s0, sc
-        const T *tk = PV == PV_GENERAL ? tab + key : lds_ptr<T>((uint32_t)key);
+        const T *tk = GEN ? tab + key : lds_ptr<T>((uint32_t)key);
         constexpr uint32_t mark = (LAW == BBPD_LAW_LINEAR) ? FAILED_BIT : FLAG_BIT;
         // the stored value: the history is its magnitude (+0 for -0), the flag its sign bit; the law
         // replaces it, and the force reads the magnitude of what is stored after the law
@@ -829,9 +829,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
             // a streamed s0 / sc enters the threshold as the region's smallest value: a stretch at or
             // below it cannot change the law's state, and the entry's own value is read only when the
             // law runs (most entries, most steps, read no parameter stream)
+            s0 = gs0;
+            sc = gsc;
+            thr = gthr;
+        } else if (PV == PV_GENTAB) {
             s0 = P.s0mode == PM_TABLE ? tab[ts0 + key] : gs0;
             sc = P.scmode == PM_TABLE ? tab[tsc + key] : gsc;
-            thr = gtab ? thr_of(law_threshold<LAW, T>(s0, sc, rs1)) : gthr;
+            thr = thr_of(law_threshold<LAW, T>(s0, sc, rs1));
         }
         // the law at stretch s once its test has passed, and the store of a changed history (sign = flag)
         auto law_run = [&](T s) {
@@ -840,8 +844,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
                 s0 = bload<T>(rtab, (uint32_t)(ts0 + ki) * TW);
                 sc = bload<T>(rtab, (uint32_t)(tsc + ki) * TW);
             }
-            if (PV == PV_GENERAL && P.s0mode == PM_STREAM) s0 = bload<T>(r0, lb + so);
-            if (PV == PV_GENERAL && P.scmode == PM_STREAM) sc = bload<T>(r1, lb + so);
+            if (GEN && P.s0mode == PM_STREAM) s0 = bload<T>(r0, lb + so);
+            if (GEN && P.scmode == PM_STREAM) sc = bload<T>(r1, lb + so);
             uint32_t w = flag ? mark : 0u;
             T H = MIXED ? (T)fabsf((float)d.h) : (T)fabs((double)d.h);
             apply_law<LAW, T>(s, s0, sc, rs1, lk, lalpha, ldenom, lsteel, lff, H, w);
@@ -875,7 +879,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TILE && O
             // buildhorizons.m:93-97: (ex^2 + ey^2) + ez^2, the y and z squares from the row
             const double ex = Xn0 - Xm0;
             double q0;
-            if (PV == PV_GENERAL) {
+            if (GEN) {
                 const double ey = Xn1 - P.hy * (double)(ky + d.st.dy()), ez = Xn2 - P.hz * (double)(kz + d.st.dz());
                 q0 = (ex * ex + ey * ey) + ez * ez;
             } else {
@@ -1732,7 +1736,8 @@ KernelFn pick_pv_st(int pv, int law, int v)
     switch (pv) {
     case PV_SCALAR: return pick_law_st<MIXED, PV_SCALAR, TILE>(law, v);
     case PV_TABLE: return pick_law_st<MIXED, PV_TABLE, TILE>(law, v);
-    default: return pick_law_st<MIXED, PV_GENERAL, TILE>(law, v);
+    case PV_GENERAL: return pick_law_st<MIXED, PV_GENERAL, TILE>(law, v);
+    default: return pick_law_st<MIXED, PV_GENTAB, TILE>(law, v);
     }
 }
 
@@ -1743,7 +1748,8 @@ KernelFn pick_bond_kernel(const bbpd_solver *s, int variant)
     if (s->stencil) {
         const int m0 = s->P.s0mode, mc = s->P.scmode;
         const int pv = (m0 == PM_SCALAR && mc == PM_SCALAR) ? PV_SCALAR
-                       : (m0 == PM_TABLE && mc == PM_TABLE) ? PV_TABLE : PV_GENERAL;
+                       : (m0 == PM_TABLE && mc == PM_TABLE) ? PV_TABLE
+                       : (m0 == PM_TABLE || mc == PM_TABLE) ? PV_GENTAB : PV_GENERAL;
         // tiles are built for the mixed form only (the FP64 form keeps the slices' reduction order,
         // which the oracle's device-order sums follow)
         if (mixed) return s->tiles ? pick_pv_st<true, true>(pv, s->law, v) : pick_pv_st<true, false>(pv, s->law, v);

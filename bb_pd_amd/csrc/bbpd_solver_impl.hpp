@@ -56,8 +56,9 @@ using T2 = T __attribute__((ext_vector_type(2)));
 
 // how a per-bond quantity reaches the kernel
 enum ParamMode { PM_SCALAR = 0, PM_TABLE = 1, PM_STREAM = 2, PM_INLINE = 3 };
-// stencil bond kernel variants by where s0 and sc come from
-enum ParamVariant { PV_SCALAR = 0, PV_TABLE = 1, PV_GENERAL = 2 };
+// stencil bond kernel variants by where s0 and sc come from (PV_GENTAB: PV_GENERAL with a key table
+// for one of them)
+enum ParamVariant { PV_SCALAR = 0, PV_TABLE = 1, PV_GENERAL = 2, PV_GENTAB = 3 };
 
 // A region is one problem instance: a block-aligned range of slots with its own stencil, tables,
 // law scalars, contact bodies, load schedule, history and step counter.  A single problem is one
