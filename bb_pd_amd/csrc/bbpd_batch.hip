@@ -20,7 +20,7 @@ int batch_rebuild(bbpd_solver *s)
 {
     const int nblk = s->nSlots / BLOCK;
     std::vector<int> blkreg(nblk, 0), grid, act;
-    std::vector<int2> se;
+    std::vector<int4> se;
     size_t maxtab = 0;
     for (int r = 0; r < s->maxRegions; r++) {
         if (!s->active[r]) continue;
@@ -29,10 +29,10 @@ int batch_rebuild(bbpd_solver *s)
             blkreg[RG.blk0 + b] = r;
             grid.push_back(RG.blk0 + b);
         }
-        if (RG.nBodies == 0) se.push_back(make_int2(r, -1));
-        else if (RG.seq_contact) se.push_back(make_int2(r, -2));
+        if (RG.nBodies == 0) se.push_back(make_int4(r, -1, 0, 0));
+        else if (RG.seq_contact) se.push_back(make_int4(r, -2, 0, 0));
         else
-            for (int b = 0; b < RG.nBodies; b++) se.push_back(make_int2(r, b));
+            for (int b = 0; b < RG.nBodies; b++) se.push_back(make_int4(r, b, RG.body0 + b, 0));
         act.push_back(r);
         maxtab = std::max(maxtab, (size_t)RG.ntab);
     }
@@ -50,7 +50,7 @@ int batch_rebuild(bbpd_solver *s)
     if (!grid.empty()) BBPD_HIP(hipMemcpy(s->d_grid, grid.data(), sizeof(int) * grid.size(), hipMemcpyHostToDevice));
     const std::vector<int> gst = stripe_list(grid);
     if (!gst.empty()) BBPD_HIP(hipMemcpy(s->d_grid_st, gst.data(), sizeof(int) * gst.size(), hipMemcpyHostToDevice));
-    if (!se.empty()) BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int2) * se.size(), hipMemcpyHostToDevice));
+    if (!se.empty()) BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int4) * se.size(), hipMemcpyHostToDevice));
     if (!act.empty()) BBPD_HIP(hipMemcpy(s->d_active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice));
     s->grid_n = (int)grid.size();
     s->grid_n_st = (int)gst.size();

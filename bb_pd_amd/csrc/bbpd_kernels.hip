@@ -1175,7 +1175,7 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
     __shared__ double vals[SE_CH * 1024];
     __shared__ int wofs[SE_CH * 16 + 1]; // per (chunk, wavefront) contact offsets, then the count
     __shared__ int last_block, last_all;
-    const int2 job = P.se[blockIdx.x];
+    const int4 job = P.se[blockIdx.x];
     const int ri = job.x, bi = job.y;
     const Region &RG = P.reg[ri];
     RegionState &RS = P.rst[ri];
@@ -1188,9 +1188,10 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     double total = 0, rs = 0;
     // one body's contact (calculatecontactforce.m:62-121): its reaction in `total`, and for body 0
-    // the static reaction in `rs`
-    auto body = [&](const int bi) {
-        const Body &B = P.bodies[RG.body0 + bi];
+    // the static reaction in `rs`; gb is the body's index in P.bodies (the job carries it, so that
+    // the body's record does not wait for the region's)
+    auto body = [&](const int bi, const int gb) {
+        const Body &B = P.bodies[gb];
         const double disp = schedule_disp(RG, RS, it);
         const double cx = B.cx, czm = B.cz + (B.moves ? disp : 0.0), R = B.R;
         const int nf = B.nFam;
@@ -1295,26 +1296,31 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
         if (bi == 0) rs = bbpd::block_sum_fixed<1024>(part, vals);
     };
     if (bi >= 0) {
-        body(bi);
+        body(bi, job.z);
     } else if (bi == -2) {
         // overlapping families: this block applies the bodies in order, as the reference's calls
         // follow one another (dynamicsolverdisplacementcontrolled.m:108-112); a member shared with an
         // earlier body is seen as that body left it (the barrier orders the block's global writes)
         for (int b = 0; b < RG.nBodies; b++) {
             total = 0;
-            body(b);
+            body(b, RG.body0 + b);
             if (threadIdx.x == 0) RS.reaction[b] = total;
             if (b == 0 && threadIdx.x == 0) RS.reaction_static = rs;
             __syncthreads();
         }
     }
+    // the block's two tickets, taken together: the region's (its last block records the region's
+    // observables) and the launch's (its last block advances the parity, which no block reads after
+    // its start)
     if (threadIdx.x == 0) {
         if (bi >= 0) RS.reaction[bi] = total;
         if (bi == 0) RS.reaction_static = rs;
         __threadfence();
         const unsigned need = (unsigned)(RG.nBodies > 0 && !RG.seq_contact ? RG.nBodies : 1);
         const unsigned ticket = atomicAdd(&RS.done, 1u);
+        const unsigned tall = atomicAdd(P.tick_done, 1u);
         last_block = (ticket == need - 1);
+        last_all = (tall == (unsigned)gridDim.x - 1);
     }
     __syncthreads();
     if (last_block && threadIdx.x == 0) {
@@ -1369,14 +1375,7 @@ __global__ __launch_bounds__(1024) void k_step_end(Params P)
         vrs->done = 0u;
         vrs->step = it;
     }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned ticket = atomicAdd(P.tick_done, 1u);
-        last_all = (ticket == (unsigned)gridDim.x - 1);
-    }
-    __syncthreads();
     if (last_all && threadIdx.x == 0) {
-        __threadfence();
         *P.tick_done = 0u;
         *P.tick = *P.tick + 1;
     }

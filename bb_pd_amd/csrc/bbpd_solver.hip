@@ -590,11 +590,11 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         const int nblk = nSlots / BLOCK;
         std::vector<int> blkreg(nblk, 0), grid(nblk);
         std::iota(grid.begin(), grid.end(), 0);
-        std::vector<int2> se;
-        if (R0.seq_contact) se.push_back(make_int2(0, -2));
+        std::vector<int4> se;
+        if (R0.seq_contact) se.push_back(make_int4(0, -2, 0, 0));
         else
             for (int bdy = 0; bdy < std::max(pr->nBodies, 1); bdy++)
-                se.push_back(make_int2(0, pr->nBodies > 0 ? bdy : -1));
+                se.push_back(pr->nBodies > 0 ? make_int4(0, bdy, R0.body0 + bdy, 0) : make_int4(0, -1, 0, 0));
         int act = 0;
         if ((rc = dalloc(&s->d_reg, 1, s->owned))) return fail(rc);
         if ((rc = dalloc(&s->d_rst, 1, s->owned))) return fail(rc);
@@ -617,7 +617,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         BBPD_HIP(hipMemcpy(s->d_grid, grid.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_grid_st, gst.data(), sizeof(int) * gst.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_active, &act, sizeof(int), hipMemcpyHostToDevice));
-        BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int2) * se.size(), hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(s->d_se, se.data(), sizeof(int4) * se.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_bodies, hb.data(), sizeof(Body) * hb.size(), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_tick, &t0, sizeof(t0), hipMemcpyHostToDevice));
         BBPD_HIP(hipMemcpy(s->d_tick_done, &u0, sizeof(u0), hipMemcpyHostToDevice));

@@ -175,7 +175,8 @@ struct Params {
     const int *blkreg;    // global block -> region
     const int *grid;      // launch block (after the XCD remap) -> global block, -1 = idle
     const Body *bodies;
-    const int2 *se;       // step-end blocks: (region, body index or -1)
+    const int4 *se;       // step-end blocks: (region, body index in the region or -1 (no bodies) or
+                          // -2 (overlapping families), the body's index in bodies, 0)
     int nse;
     long long *tick;      // launches completed: the position-buffer parity
     unsigned int *tick_done; // step-end blocks finished this launch
@@ -253,7 +254,7 @@ struct bbpd_solver {
     int *d_blkreg = nullptr, *d_grid = nullptr, *d_active = nullptr;
     int *d_grid_st = nullptr; // the launch list in XCD stripes (launch variants 4 and 5)
     int grid_n_st = 0;
-    int2 *d_se = nullptr;
+    int4 *d_se = nullptr;
     Body *d_bodies = nullptr;
     long long *d_tick = nullptr;
     unsigned *d_tick_done = nullptr;
