@@ -764,3 +764,56 @@ def test_overlapping_contact_families_in_a_batch(gpu):
         hs = s.history()
         for k in ("reaction", "reaction_static", "deflection", "cmod"):
             assert same(hs[k], h[k]), k
+
+
+@pytest.mark.parametrize("law", [_lib.LAW_BILINEAR, _lib.LAW_EXPONENTIAL])
+def test_stencil_tabled_and_streamed_parameters(gpu, law):
+    """s0 a function of the bond's key (the pair of family sizes, here through BONDSTIFFNESS) and sc
+    per bond: the stencil layout keeps s0 in its key table and streams sc, and an entry's law
+    threshold comes from its tabled s0 and the region's smallest sc.  The stream count shows in the
+    kernel's bytes: none when both are functions of the key, one here, two when neither is.  The FP64
+    trajectory equals the record layout's bit for bit; the mixed form's stays within rounding of it."""
+    import copy
+    base = ec_case(law)
+    nB = base.nBonds
+    c = np.asarray(base.c, dtype=np.float64)
+    rng = np.random.default_rng(7)
+    s0_key = 1e-4 * (1 + 0.02 * c / c.max())
+    sc0 = np.asarray(base.sc, dtype=np.float64) if np.ndim(base.sc) else np.full(nB, float(base.sc))
+    sc_key = sc0.max() * (1 + 0.03 * c / c.max())
+    sc_bond = sc0 * (1 + 0.05 * rng.random(nB))
+    s0_bond = s0_key * (1 + 0.01 * (2 * rng.random(nB) - 1))
+
+    def variant(s0, sc):
+        v = copy.copy(base)
+        v.s0, v.sc = s0, sc
+        return v
+
+    mixed_case = variant(s0_key, sc_bond)
+    by = {}
+    for name, case in (("table", variant(s0_key, sc_key)), ("one stream", mixed_case),
+                       ("two streams", variant(s0_bond, sc_bond))):
+        with _solver_with_layout(case, _lib.FP64, "stencil") as s:
+            assert s.layout() == "stencil", s.layout()
+            inf = s.info()
+            by[name] = (inf["bond_kernel_bytes"], inf["n_entries"])
+    ne = by["table"][1]
+    assert by["one stream"][0] - by["table"][0] == 8 * ne
+    assert by["two streams"][0] - by["one stream"][0] == 8 * ne
+    print(f"case: law {law}, s0 tabled, sc streamed, {nB} bonds", flush=True)
+    a = _solver_with_layout(mixed_case, _lib.FP64, "stencil")
+    b = _solver_with_layout(mixed_case, _lib.FP64, "records")
+    m = _solver_with_layout(mixed_case, _lib.MIXED, "stencil")
+    with a, b, m:
+        for s in (a, b, m):
+            s.step(300)
+        for x, y in zip(a.nodal(), b.nodal()):
+            assert same(x, y)
+        for x, y in zip(a.bonds(), b.bonds()):
+            assert same(x, y)
+        assert same(a.history()["reaction"], b.history()["reaction"])
+        assert a.observables()["n_softening"] > 0
+        ua, um = a.nodal()[0], m.nodal()[0]
+        err = np.abs(ua - um).max() / np.abs(ua).max()
+        print(f"  mixed vs FP64: u {err:.2e}", flush=True)
+        assert err < 2e-3
