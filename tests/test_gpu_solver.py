@@ -576,12 +576,13 @@ def test_stencil_layout_restore_and_adr(gpu):
             assert same(x, y)
 
 
-def test_stencil_layout_mixed_close_to_records(gpu):
+@pytest.mark.parametrize("law", [_lib.LAW_BILINEAR, _lib.LAW_TRILINEAR, _lib.LAW_EXPONENTIAL, _lib.LAW_LINEAR])
+def test_stencil_layout_mixed_close_to_records(gpu, law):
     """Mixed form: the two layouts round differently (c*V*vcf and xi; the stencil kernel takes the
-    length through a reciprocal square root and adds pairs of entries in FP32 before the FP64 row
-    sum), nothing else: after 300 Euler-Cromer steps both stay as close to the FP64 run as to each
-    other."""
-    case = ec_case(_lib.LAW_BILINEAR, steps=300)
+    stretch and the force factor from series in (|y|^2 - |xi|^2) / (2 |xi|^2), tests the law's threshold
+    in that variable, and adds groups of entries in FP32 before the FP64 row sum), nothing else: after
+    300 Euler-Cromer steps, for each law, both stay as close to the FP64 run as to each other."""
+    case = ec_case(law, steps=300)
     a = _solver_with_layout(case, _lib.MIXED, "stencil")
     b = _solver_with_layout(case, _lib.MIXED, "records")
     c = _solver_with_layout(case, _lib.FP64, "records")
@@ -596,6 +597,12 @@ def test_stencil_layout_mixed_close_to_records(gpu):
         assert d_ab <= 1e-4 and d_ac <= 1e-4 and d_bc <= 1e-4
         fa, fb = a.bonds()[1], b.bonds()[1]
         assert abs(int(fa.sum()) - int(fb.sum())) <= max(3, 0.01 * fb.sum())
+        na, nb = int((a.bonds()[2] == 0).sum()), int((b.bonds()[2] == 0).sum())
+        assert abs(na - nb) <= max(3, 0.01 * nb)
+        if law == _lib.LAW_LINEAR:
+            assert nb > 0
+        else:
+            assert fb.sum() > 0
 
 
 def test_layout_defaults_and_fallback(gpu):
