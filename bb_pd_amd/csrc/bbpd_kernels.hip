@@ -258,6 +258,7 @@ struct Ent {
     uint4 r;             // entry record
     T p0, p1;            // streamed per-bond s0, sc
     double c;            // streamed BONDSTIFFNESS (FP64 form)
+    double L0, vcf;      // streamed UNDEFORMEDLENGTH, VOLUMECORRECTIONFACTORS (FP64 form, Params::L0str)
     NodeRec nb;          // neighbour record
     double b0, b1, b2;   // neighbour undeformed coordinates (off-lattice inputs)
 };
@@ -305,6 +306,10 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
         if (P.s0mode == PM_STREAM) d.p0 = __builtin_nontemporal_load((const T *)P.s0str + e);
         if (P.scmode == PM_STREAM) d.p1 = __builtin_nontemporal_load((const T *)P.scstr + e);
         if (!MIXED && P.cmode == PM_STREAM) d.c = __builtin_nontemporal_load(P.cstr + e);
+        if (!MIXED && P.L0str) {
+            d.L0 = __builtin_nontemporal_load(P.L0str + e);
+            d.vcf = __builtin_nontemporal_load(P.vcfstr + e);
+        }
         const int m = (int)(w & NODE_MASK);
         d.nb = pos[m];
         if (!LAT) {
@@ -333,13 +338,16 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             // reference's x_j - x_i at the j end, so |.|, s and the force magnitude are identical
             const double dx = d.nb.x - qn0, dy = d.nb.y - qn1, dz = d.nb.z - qn2;
             const double l = sqrt(dx * dx + dy * dy + dz * dz);
+            // the undeformed length and volume correction factor: as given where the problem's arrays
+            // differ from the formulas, else recomputed (buildhorizons.m:93-97,
+            // calculatevolumecorrectionfactors.m:20-32)
             const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
-            const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+            const double L0 = P.L0str ? d.L0 : sqrt((ex * ex + ey * ey) + ez * ez);
             const double s = (l - L0) / L0;
             apply_law<LAW, double>(s, s0, sc, rs1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
-            // calculatevolumecorrectionfactors.m:20-32
             double vcf;
-            if (L0 <= P.hmr) vcf = 1;
+            if (P.L0str) vcf = d.vcf;
+            else if (L0 <= P.hmr) vcf = 1;
             else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
             else vcf = 0;
             // calculatebondforcesMex.c:34-40 with stretchPlastic = 0 (s - 0 == s exactly)
@@ -1509,8 +1517,10 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
         // deformed coordinates: the FP64 form stores them, the mixed form stores u (x = X + u)
         const double xn0 = MIXED ? Xn[0] + me.x : me.x, xn1 = MIXED ? Xn[1] + me.y : me.y,
                      xn2 = MIXED ? Xn[2] + me.z : me.z;
+        // gL0 / gvcf: the given undeformed length and volume correction factor (streamed, FP64 record
+        // layout), or negative to recompute them from the coordinates
         auto bond = [&](double H, bool failed, int m, double c, double s0, double sc, double Xm0, double Xm1,
-                        double Xm2) {
+                        double Xm2, double gL0 = -1.0, double gvcf = -1.0) {
             const bool broken = failed || H == 1.0;
             cnt++;
             unbroken += broken ? 0 : 1;
@@ -1519,14 +1529,15 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
                 // calculatevolumecorrectionfactors.m:20-32, then the micropotential of
                 // calculatestrainenergy.m:16 (BFMULTIPLIER = 1 for concrete, buildbonddata.m:56-57)
                 const double ex = Xn[0] - Xm0, ey = Xn[1] - Xm1, ez = Xn[2] - Xm2;
-                const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+                const double L0 = gL0 >= 0 ? gL0 : sqrt((ex * ex + ey * ey) + ez * ez);
                 const NodeRec pm = pos[m];
                 const double dx = (MIXED ? Xm0 + pm.x : pm.x) - xn0, dy = (MIXED ? Xm1 + pm.y : pm.y) - xn1,
                              dz = (MIXED ? Xm2 + pm.z : pm.z) - xn2;
                 const double l = sqrt(dx * dx + dy * dy + dz * dz);
                 const double st = (l - L0) / L0;
                 double vcf;
-                if (L0 <= P.hmr) vcf = 1;
+                if (gvcf >= 0) vcf = gvcf;
+                else if (L0 <= P.hmr) vcf = 1;
                 else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
                 else vcf = 0;
                 const double fl = (LAW == BBPD_LAW_LINEAR && failed) ? 0.0 : 1.0;
@@ -1534,7 +1545,7 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
             }
             if (m > n && LAW != BBPD_LAW_LINEAR) { // each bond once, at its lower slot
                 const double ex = Xn[0] - Xm0, ey = Xn[1] - Xm1, ez = Xn[2] - Xm2;
-                const double L0 = sqrt((ex * ex + ey * ey) + ez * ez);
+                const double L0 = gL0 >= 0 ? gL0 : sqrt((ex * ex + ey * ey) + ez * ez);
                 const double smax = max_stretch<LAW>(H, s0, sc, RG.s1, P.k, P.alpha, P.denom_exp);
                 e_acc += damage_energy(c, s0, sc, L0, smax, P.k, P.alpha, P.V);
             }
@@ -1608,7 +1619,8 @@ __global__ __launch_bounds__(BLOCK) void k_damage(Params P, int ri, double *dmg,
                     else vcf = 0;
                     c = vcf > 0 ? (double)__uint_as_float(r.w) / (P.V * vcf) : 0.0;
                 }
-                bond(H, (w & FAILED_BIT) != 0, m, c, s0, sc, Xm[0], Xm[1], Xm[2]);
+                if (!MIXED && P.L0str) bond(H, (w & FAILED_BIT) != 0, m, c, s0, sc, Xm[0], Xm[1], Xm[2], P.L0str[e], P.vcfstr[e]);
+                else bond(H, (w & FAILED_BIT) != 0, m, c, s0, sc, Xm[0], Xm[1], Xm[2]);
             }
         }
         if (dmg) dmg[n - RG.base] = cnt > 0 ? 1 - ((double)unbroken / (double)cnt) : 0.0;

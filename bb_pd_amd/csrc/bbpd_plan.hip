@@ -47,15 +47,12 @@ bool make_table(const double *a, int n, int cap, std::vector<double> &tab, std::
     return true;
 }
 
-
-// Checks of a problem's bond arrays: ids, UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS against their
-// formulas (both are recomputed on the device: buildhorizons.m:93-97,
-// calculatevolumecorrectionfactors.m:20-32), fail and BONDTYPE values.  Fills the 0-based ends.
 namespace {
 
 // The checks of bonds [k0, k1) (see check_bonds): BBPD_OK, or the error code of the first that fails,
-// with the message set when `report`.
-int check_bond_range(const bbpd_problem *pr, int k0, int k1, int *bi, int *bj, bool report)
+// with the message set when `report`.  With `geom`, a length or volume correction factor that differs
+// from its formula sets *geom instead of failing.
+int check_bond_range(const bbpd_problem *pr, int k0, int k1, int *bi, int *bj, bool report, int *geom)
 {
     const int nN = pr->nNodes, nB = pr->nBonds;
     const double *X = pr->undeformedCoordinates, *BL = pr->BONDLIST;
@@ -74,7 +71,9 @@ int check_bond_range(const bbpd_problem *pr, int k0, int k1, int *bi, int *bj, b
         double ex = X[i - 1] - X[j - 1], ey = X[i - 1 + nN] - X[j - 1 + nN],
                ez = X[i - 1 + 2 * (size_t)nN] - X[j - 1 + 2 * (size_t)nN];
         double L0 = std::sqrt((ex * ex + ey * ey) + ez * ez);
-        if (!bits_equal(L0, pr->UNDEFORMEDLENGTH[k])) {
+        if (!bits_equal(L0, pr->UNDEFORMEDLENGTH[k]) && geom) {
+            *geom = 1;
+        } else if (!bits_equal(L0, pr->UNDEFORMEDLENGTH[k])) {
             if (report)
                 bbpd::set_error("UNDEFORMEDLENGTH(" + std::to_string(k + 1) +
                                 ") differs from the length recomputed from undeformedCoordinates (buildhorizons.m:93-97)");
@@ -84,7 +83,9 @@ int check_bond_range(const bbpd_problem *pr, int k0, int k1, int *bi, int *bj, b
         if (L0 <= hmr) vcf = 1;
         else if ((L0 > hmr) && (L0 <= horizon)) vcf = (hpr - L0) / two_r;
         else vcf = 0;
-        if (!bits_equal(vcf, pr->VOLUMECORRECTIONFACTORS[k])) {
+        if (!bits_equal(vcf, pr->VOLUMECORRECTIONFACTORS[k]) && geom) {
+            *geom = 1;
+        } else if (!bits_equal(vcf, pr->VOLUMECORRECTIONFACTORS[k])) {
             if (report)
                 bbpd::set_error("VOLUMECORRECTIONFACTORS(" + std::to_string(k + 1) +
                                 ") differs from calculatevolumecorrectionfactors(L0, horizon, RADIJ)");
@@ -105,32 +106,35 @@ int check_bond_range(const bbpd_problem *pr, int k0, int k1, int *bi, int *bj, b
 } // namespace
 
 // Checks of a problem's bond arrays: ids, UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS against their
-// formulas (both are recomputed on the device: buildhorizons.m:93-97,
-// calculatevolumecorrectionfactors.m:20-32), fail and BONDTYPE values.  Fills the 0-based ends.  Large
-// problems are checked in chunks by several host threads; the first failing bond in bond order is
-// the one reported.
-int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj)
+// formulas (buildhorizons.m:93-97, calculatevolumecorrectionfactors.m:20-32: the device recomputes
+// both from the coordinates), fail and BONDTYPE values.  Fills the 0-based ends.  A length or factor
+// that differs from its formula is an error (BBPD_ERR_UNSUPPORTED), or, with geom_differs, sets
+// *geom_differs (the caller then streams the given arrays).  Large problems are checked in chunks by
+// several host threads; the first failing bond in bond order is the one reported.
+int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj, bool *geom_differs)
 {
     const int nB = pr->nBonds;
     bi.assign(nB, 0);
     bj.assign(nB, 0);
     const int T = nB < (1 << 20) ? 1 : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     const int chunk = (nB + T - 1) / T;
-    std::vector<int> rc(T, BBPD_OK);
+    std::vector<int> rc(T, BBPD_OK), geom(T, 0);
+    int *const g = geom_differs ? geom.data() : nullptr;
     std::vector<std::thread> th;
     for (int t = 1; t < T; t++)
         th.emplace_back([&, t] {
             rc[t] = check_bond_range(pr, std::min(nB, t * chunk), std::min(nB, (t + 1) * chunk), bi.data(), bj.data(),
-                                     false);
+                                     false, g ? g + t : nullptr);
         });
-    rc[0] = check_bond_range(pr, 0, std::min(nB, chunk), bi.data(), bj.data(), true);
+    rc[0] = check_bond_range(pr, 0, std::min(nB, chunk), bi.data(), bj.data(), true, g);
     for (auto &x : th) x.join();
     for (int t = 0; t < T; t++)
         if (rc[t] != BBPD_OK) {
             // the first failing chunk again, on this thread, for its message
             return check_bond_range(pr, std::min(nB, t * chunk), std::min(nB, (t + 1) * chunk), bi.data(), bj.data(),
-                                    true);
+                                    true, g ? g + t : nullptr);
         }
+    if (geom_differs) *geom_differs = std::find(geom.begin(), geom.end(), 1) != geom.end();
     return BBPD_OK;
 }
 

@@ -54,8 +54,12 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     const double horizon = pr->horizon, RADIJ = pr->RADIJ;
     const double hmr = horizon - RADIJ, hpr = horizon + RADIJ, two_r = 2 * RADIJ;
     std::vector<int> bi, bj;
+    // FP64 form: UNDEFORMEDLENGTH or VOLUMECORRECTIONFACTORS other than their formulas' values are
+    // streamed per entry on the record layout (both layouts otherwise recompute them from X); the mixed
+    // form, whose stretch is formed from the coordinates, refuses them
+    bool geom_differs = false;
     {
-        const int rc0 = check_bonds(pr, bi, bj);
+        const int rc0 = check_bonds(pr, bi, bj, mixed ? nullptr : &geom_differs);
         if (rc0) return fail(rc0);
     }
     double xmin[3], xmax[3], hq, hlat[3];
@@ -70,6 +74,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         const char *lay = std::getenv("BBPD_LAYOUT");
         const bool want_records = lay && std::strcmp(lay, "records") == 0;
         if (!lattice) sp.why = "not a lattice";
+        else if (geom_differs) sp.why = "UNDEFORMEDLENGTH or VOLUMECORRECTIONFACTORS differ from their formulas";
         else if (want_records) sp.why = "record layout requested";
         else plan_stencil(pr, X, bi, bj, hlat, sp);
     }
@@ -182,6 +187,11 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     std::vector<double> h_c64, h_s064, h_sc64;
     std::vector<float> h_s032, h_sc032;
     if (!mixed && cmode == PM_STREAM) h_c64.assign(nE, 0.0);
+    std::vector<double> h_L0, h_vcf;
+    if (geom_differs) {
+        h_L0.assign(nE, 0.0);
+        h_vcf.assign(nE, 0.0);
+    }
     if (s0mode == PM_STREAM) {
         if (mixed) h_s032.assign(nE, 0.f);
         else h_s064.assign(nE, 0.0);
@@ -231,6 +241,10 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
                 if (cmode == PM_STREAM) h_c64[e] = pr->BONDSTIFFNESS[k];
                 if (s0mode == PM_STREAM) h_s064[e] = pr->s0[k];
                 if (scmode == PM_STREAM) h_sc64[e] = pr->sc[k];
+                if (geom_differs) {
+                    h_L0[e] = pr->UNDEFORMEDLENGTH[k];
+                    h_vcf[e] = pr->VOLUMECORRECTIONFACTORS[k];
+                }
             } else {
                 float pin = 0.f;
                 if (s0mode == PM_INLINE) pin = (float)pr->s0[k];
@@ -301,6 +315,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     void *s0d = nullptr, *scd = nullptr;
     const size_t ew = mixed ? 4 : 8;
     if (!h_c64.empty() && (rc = dalloc(&cstrd, nE, s->owned))) return fail(rc);
+    double *L0d = nullptr, *vcfd = nullptr;
+    if (geom_differs && ((rc = dalloc(&L0d, nE, s->owned)) || (rc = dalloc(&vcfd, nE, s->owned)))) return fail(rc);
     if (s0mode == PM_STREAM && (rc = dalloc((char **)&s0d, (nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     if (scmode == PM_STREAM && (rc = dalloc((char **)&scd, (nE + 2 * SLICE) * ew, s->owned))) return fail(rc);
     const size_t ntab = tabc.size() + tabp.size();
@@ -414,6 +430,10 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         }
     }
     if (cstrd) BBPD_HIP(hipMemcpy(cstrd, h_c64.data(), 8 * nE, hipMemcpyHostToDevice));
+    if (L0d) {
+        BBPD_HIP(hipMemcpy(L0d, h_L0.data(), 8 * nE, hipMemcpyHostToDevice));
+        BBPD_HIP(hipMemcpy(vcfd, h_vcf.data(), 8 * nE, hipMemcpyHostToDevice));
+    }
     if (s0d) BBPD_HIP(hipMemcpy(s0d, mixed ? (void *)h_s032.data() : (void *)h_s064.data(), ew * nE, hipMemcpyHostToDevice));
     if (scd) BBPD_HIP(hipMemcpy(scd, mixed ? (void *)h_sc032.data() : (void *)h_sc64.data(), ew * nE, hipMemcpyHostToDevice));
     {
@@ -452,6 +472,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     P.slice_off = soffd;
     P.rec = recd;
     P.cstr = cstrd;
+    P.L0str = L0d;
+    P.vcfstr = vcfd;
     P.s0str = s0d;
     P.scstr = scd;
     P.tab = tabd;
@@ -644,7 +666,7 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     s->device_bytes = sizeof(NodeRec) * (size_t)nSlots * (lattice ? 2 : 3) +
                       sizeof(double) * (size_t)nSlots * (3 * (mixed ? 3 : 4) + 2 + (Fpd ? 3 : 0)) + nSlots +
                       (sp.ok ? (mixed ? 4 : 8) * (size_t)nE + sizeof(uint4) * (size_t)nSlots : sizeof(uint4) * (size_t)nE) +
-                      (cstrd ? 8 * nE : 0) + (s0d ? ew * nE : 0) + (scd ? ew * nE : 0);
+                      (cstrd ? 8 * nE : 0) + (s0d ? ew * nE : 0) + (scd ? ew * nE : 0) + (L0d ? 16 * nE : 0);
 
     s->ev.resize(2048);
     for (auto &e : s->ev) BBPD_HIP(hipEventCreate(&e));
@@ -1220,7 +1242,7 @@ extern "C" int bbpd_solver_info(bbpd_solver *s, int64_t *device_bytes, int64_t *
     const int64_t w = mixed ? 4 : 8;
     // algorithmic bytes of one bond-kernel launch as laid out here: per entry the 16-byte record
     // plus any streamed per-bond quantity; per node the state the kernel reads and writes
-    int64_t per_entry = (s->stencil ? w : 16) + (s->P.cmode == PM_STREAM && !mixed ? 8 : 0) +
+    int64_t per_entry = (s->stencil ? w : 16) + (s->P.cmode == PM_STREAM && !mixed ? 8 : 0) + (s->P.L0str ? 16 : 0) +
                         (s->P.s0mode == PM_STREAM ? w : 0) + (s->P.scmode == PM_STREAM ? w : 0);
     int64_t per_node;
     if (s->integrator == BBPD_INTEGRATOR_ADR)

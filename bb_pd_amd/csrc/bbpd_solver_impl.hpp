@@ -154,6 +154,9 @@ struct Params {
     const long long *slice_off;
     uint4 *rec;
     const double *cstr;          // FP64: BONDSTIFFNESS per entry when not tabled
+    const double *L0str, *vcfstr; // FP64 record layout: UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS per
+                                  // entry where the given arrays differ from their formulas (else
+                                  // nullptr: both recomputed from the coordinates)
     const void *s0str, *scstr;   // per-entry s0 / sc when streamed (FP64 double, mixed float)
     const double *tab;           // FP64 tables: c values, then parameter values
     int ntabc, ntabp;
@@ -379,7 +382,7 @@ void launch_damage(bbpd_solver *s, int ri, double *dmg, double *epart, double *s
 // ---- bbpd_plan.hip: set-up checks and the stencil plan
 bool bits_equal(double a, double b);
 bool make_table(const double *a, int n, int cap, std::vector<double> &tab, std::vector<uint32_t> &idx);
-int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj);
+int check_bonds(const bbpd_problem *pr, std::vector<int> &bi, std::vector<int> &bj, bool *geom_differs = nullptr);
 bool detect_lattice(const bbpd_problem *pr, double hlat[3], double xmin[3], double xmax[3], double &hq);
 std::vector<unsigned long long> lane_masks(const std::vector<uint4> &m, long long nslots, int noff);
 std::vector<unsigned long long> tile_lane_masks(const std::vector<uint4> &m, long long ntiles, int noff, int nx);

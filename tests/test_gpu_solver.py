@@ -824,3 +824,39 @@ def test_stencil_tabled_and_streamed_parameters(gpu, law):
         err = np.abs(ua - um).max() / np.abs(ua).max()
         print(f"  mixed vs FP64: u {err:.2e}", flush=True)
         assert err < 2e-3
+
+
+def test_fp64_given_lengths_and_factors_streamed(gpu, oracle):
+    """UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS that differ from their formulas (one ulp on every
+    third bond's length, a relative 2^-30 on every fifth partial factor) are taken as given: the FP64
+    form runs the problem on the record layout with both streamed per entry, bit for bit with the
+    oracle, which uses the given arrays (trajectory, strain energy); the mixed form, whose stretch
+    comes from the coordinates, refuses them."""
+    case = ec_case(_lib.LAW_BILINEAR, steps=300)
+    nB = case.nBonds
+    k = np.arange(nB)
+    case.L0 = np.asarray(case.L0, dtype=np.float64).copy()
+    case.L0[k % 3 == 1] = np.nextafter(case.L0[k % 3 == 1], np.inf)
+    case.vcf = np.asarray(case.vcf, dtype=np.float64).copy()
+    part = (k % 5 == 2) & (case.vcf < 1) & (case.vcf > 0)
+    assert part.sum() > 0
+    case.vcf[part] = case.vcf[part] * (1 + 2.0 ** -30)
+    print(f"case: given lengths and factors, {nB} bonds", flush=True)
+    run = oracle.OracleRun(case)
+    with Solver(case, _lib.FP64) as sol:
+        assert sol.layout() == "records (UNDEFORMEDLENGTH or VOLUMECORRECTIONFACTORS differ from their formulas)", \
+            sol.layout()
+        for blk in range(3):
+            sol.step(100)
+            run.step(100)
+            compare_state(sol, run, exact=True)
+        assert same(sol.history()["reaction"], run.reactions[-300:])
+        l, dX, dY, dZ, s = (np.zeros(nB) for _ in range(5))
+        oracle.calculatedeformedlength(run.x, case.BONDLIST, case.L0, l, dX, dY, dZ, s)
+        e_ref, d_ref = oracle.calculatestrainenergy(case.nNodes, run.fail, case.c, s, case.L0, case.vcf,
+                                                    case.BONDLIST, case.cellVolume)
+        d, e = sol.strain_energy()
+        assert same(d, d_ref) and same([e], [e_ref])
+        assert sol.observables()["n_softening"] > 0
+    with pytest.raises(_lib.BBPDError, match="UNDEFORMEDLENGTH"):
+        Solver(case, _lib.MIXED)
