@@ -263,7 +263,7 @@ struct Ent {
     double b0, b1, b2;   // neighbour undeformed coordinates (off-lattice inputs)
 };
 
-template <bool MIXED, int LAW, bool LAT>
+template <bool MIXED, int LAW, bool LAT, bool GIVEN = false>
 __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
         if (P.s0mode == PM_STREAM) d.p0 = __builtin_nontemporal_load((const T *)P.s0str + e);
         if (P.scmode == PM_STREAM) d.p1 = __builtin_nontemporal_load((const T *)P.scstr + e);
         if (!MIXED && P.cmode == PM_STREAM) d.c = __builtin_nontemporal_load(P.cstr + e);
-        if (!MIXED && P.L0str) {
+        if (GIVEN) {
             d.L0 = __builtin_nontemporal_load(P.L0str + e);
             d.vcf = __builtin_nontemporal_load(P.vcfstr + e);
         }
@@ -342,11 +342,11 @@ __global__ __launch_bounds__(BLOCK) void k_bond(Params P)
             // differ from the formulas, else recomputed (buildhorizons.m:93-97,
             // calculatevolumecorrectionfactors.m:20-32)
             const double ex = Xn0 - Xm0, ey = Xn1 - Xm1, ez = Xn2 - Xm2;
-            const double L0 = P.L0str ? d.L0 : sqrt((ex * ex + ey * ey) + ez * ez);
+            const double L0 = GIVEN ? d.L0 : sqrt((ex * ex + ey * ey) + ez * ez);
             const double s = (l - L0) / L0;
             apply_law<LAW, double>(s, s0, sc, rs1, P.k, P.alpha, P.denom_exp, P.sc_steel, P.ff, H, w);
             double vcf;
-            if (P.L0str) vcf = d.vcf;
+            if (GIVEN) vcf = d.vcf;
             else if (L0 <= P.hmr) vcf = 1;
             else if ((L0 > P.hmr) && (L0 <= P.horizon)) vcf = (P.hpr - L0) / P.two_r;
             else vcf = 0;
@@ -1661,19 +1661,22 @@ void launch_damage(bbpd_solver *s, int ri, double *dmg, double *epart, double *s
 }
 
 template <bool MIXED, int LAW>
-KernelFn pick_lat(bool lat)
+KernelFn pick_lat(bool lat, bool given)
 {
+    // given lengths and factors (Params::L0str): the FP64 form, off-lattice inputs only
+    if constexpr (!MIXED)
+        if (given) return k_bond<false, LAW, false, true>;
     return lat ? k_bond<MIXED, LAW, true> : k_bond<MIXED, LAW, false>;
 }
 
 template <bool MIXED>
-KernelFn pick_law(int law, bool lat)
+KernelFn pick_law(int law, bool lat, bool given)
 {
     switch (law) {
-    case BBPD_LAW_LINEAR: return pick_lat<MIXED, BBPD_LAW_LINEAR>(lat);
-    case BBPD_LAW_BILINEAR: return pick_lat<MIXED, BBPD_LAW_BILINEAR>(lat);
-    case BBPD_LAW_TRILINEAR: return pick_lat<MIXED, BBPD_LAW_TRILINEAR>(lat);
-    default: return pick_lat<MIXED, BBPD_LAW_EXPONENTIAL>(lat);
+    case BBPD_LAW_LINEAR: return pick_lat<MIXED, BBPD_LAW_LINEAR>(lat, given);
+    case BBPD_LAW_BILINEAR: return pick_lat<MIXED, BBPD_LAW_BILINEAR>(lat, given);
+    case BBPD_LAW_TRILINEAR: return pick_lat<MIXED, BBPD_LAW_TRILINEAR>(lat, given);
+    default: return pick_lat<MIXED, BBPD_LAW_EXPONENTIAL>(lat, given);
     }
 }
 
@@ -1766,7 +1769,7 @@ KernelFn pick_bond_kernel(const bbpd_solver *s, int variant)
         if (mixed) return s->tiles ? pick_pv_st<true, true>(pv, s->law, v) : pick_pv_st<true, false>(pv, s->law, v);
         return pick_pv_st<false, false>(pv, s->law, v);
     }
-    return mixed ? pick_law<true>(s->law, s->lattice) : pick_law<false>(s->law, s->lattice);
+    return mixed ? pick_law<true>(s->law, s->lattice, false) : pick_law<false>(s->law, s->lattice, s->P.L0str != nullptr);
 }
 
 // Let every launch variant of the solver's bond kernel take its key tables in dynamic LDS beside

@@ -63,7 +63,9 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
         if (rc0) return fail(rc0);
     }
     double xmin[3], xmax[3], hq, hlat[3];
-    const bool lattice = detect_lattice(pr, hlat, xmin, xmax, hq);
+    // given lengths and factors run as off-lattice inputs (neighbour coordinates from records): the
+    // bond kernel's variant for those carries the two streams
+    const bool lattice = detect_lattice(pr, hlat, xmin, xmax, hq) && !geom_differs;
     s->lattice = lattice;
 
     // ---- layout.  Both forms take the stencil layout where it applies: it moves a fraction of the
@@ -73,8 +75,8 @@ extern "C" int bbpd_solver_create(const bbpd_problem *pr, int precision, int dev
     {
         const char *lay = std::getenv("BBPD_LAYOUT");
         const bool want_records = lay && std::strcmp(lay, "records") == 0;
-        if (!lattice) sp.why = "not a lattice";
-        else if (geom_differs) sp.why = "UNDEFORMEDLENGTH or VOLUMECORRECTIONFACTORS differ from their formulas";
+        if (geom_differs) sp.why = "UNDEFORMEDLENGTH or VOLUMECORRECTIONFACTORS differ from their formulas";
+        else if (!lattice) sp.why = "not a lattice";
         else if (want_records) sp.why = "record layout requested";
         else plan_stencil(pr, X, bi, bj, hlat, sp);
     }

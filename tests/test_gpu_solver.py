@@ -357,13 +357,18 @@ def test_node_order_does_not_matter(gpu, oracle):
         compare_state(sol, run, exact=True)
 
 
-def test_setup_rejects_inconsistent_inputs(gpu):
+def test_setup_inconsistent_lengths(gpu):
+    """An UNDEFORMEDLENGTH that differs from its formula: the FP64 form takes it as given (record
+    layout, streamed; its trajectory is checked against the oracle below), the mixed form refuses the
+    problem and names the bond."""
     case = ec_case(_lib.LAW_BILINEAR)
     bad = case.L0.copy()
     bad[5] *= 1 + 1e-15
     case.L0 = bad
-    with pytest.raises(_lib.BBPDError, match="UNDEFORMEDLENGTH"):
-        Solver(case, _lib.FP64)
+    with Solver(case, _lib.FP64) as s:
+        assert s.layout() == "records (UNDEFORMEDLENGTH or VOLUMECORRECTIONFACTORS differ from their formulas)"
+    with pytest.raises(_lib.BBPDError, match=r"UNDEFORMEDLENGTH\(6\)"):
+        Solver(case, _lib.MIXED)
 
 
 def test_mixed_precision_error_report(gpu):
