@@ -46,15 +46,38 @@ __device__ __forceinline__ T ldiv(T a, T b)
         return a / b;
 }
 
-// e^x: in FP64 the library exponential; in FP32 (the mixed form) 2^(x log2 e) by the hardware
-// exponential.  For the law's arguments (x in [-k, 0]) its absolute error stays at the FP32 level.
+// e^x in FP64 for the law's arguments (x in [-k, 0], k = 25 in the reference; any x in [-700, 700]):
+// x = n ln2 + r with n = rint(x / ln2) and r by a two-part reduction (ln2 = LN2_HI + LN2_LO, LN2_HI with
+// 21 significant bits, so n LN2_HI and x - n LN2_HI are exact), |r| <= ln2 / 2; e^r by its Taylor
+// polynomial to degree 13 (truncation below 2^-60 of the value) in Horner form; then the scaling by
+// 2^n.  Within about one ulp of e^x.  The coefficients come from constant memory, so the compiler
+// keeps them in scalar registers rather than in vector registers it would spill under pressure.
+__constant__ double EXP_TAYLOR[14] = {
+    1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333, 0.001388888888888889,
+    0.0001984126984126984, 2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07,
+    2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10};
+__device__ __forceinline__ double exp_law(double x)
+{
+    constexpr double LOG2E = 1.4426950408889634;    // 1 / ln2
+    constexpr double LN2_HI = 0x1.62e42p-1;         // ln2 to 21 significant bits
+    constexpr double LN2_LO = 0x1.fdf473de6af28p-22; // ln2 - LN2_HI
+    const double n = rint(x * LOG2E);
+    const double r = fma(-n, LN2_LO, fma(-n, LN2_HI, x));
+    double p = EXP_TAYLOR[13];
+#pragma unroll
+    for (int i = 12; i >= 0; i--) p = fma(p, r, EXP_TAYLOR[i]);
+    return ldexp(p, (int)n);
+}
+
+// e^x: in FP64 exp_law; in FP32 (the mixed form) 2^(x log2 e) by the hardware exponential.  For the
+// law's arguments (x in [-k, 0]) the FP32 form's absolute error stays at the FP32 level.
 template <typename T>
 __device__ __forceinline__ T lexp(T x)
 {
     if constexpr (std::is_same<T, float>::value)
         return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
     else
-        return exp(x);
+        return exp_law(x);
 }
 
 // The laws.  A bond whose history is already 1 (broken) keeps it: every candidate is clamped to 1
