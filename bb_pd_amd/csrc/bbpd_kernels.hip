@@ -24,8 +24,11 @@
 //     per-bond s0 / sc with few distinct values live in small tables copied to LDS by each block;
 //     otherwise they are streamed from their own arrays.
 //   * UNDEFORMEDLENGTH and VOLUMECORRECTIONFACTORS are recomputed from the coordinates with the
-//     reference formulas (buildhorizons.m:93-97, calculatevolumecorrectionfactors.m:16-34); set-up
-//     checks bit for bit that the given arrays equal them.
+//     reference formulas (buildhorizons.m:93-97, calculatevolumecorrectionfactors.m:16-34) when set-up
+//     finds, bit for bit, that the given arrays equal them; otherwise (FP64 form) both are streamed
+//     per entry to the off-lattice record kernel's variant for given values.
+//   * Stencil layout (lattice boxes; see k_bond_st): neighbours implicit by slot offset, per entry only
+//     the history.
 //   * A step reads position buffer step&1 and writes the other, so the Euler-Cromer update of a node
 //     runs in the kernel that gathers its neighbours.
 #include "bbpd_solver_impl.hpp"
