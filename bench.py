@@ -251,14 +251,14 @@ def one_step_from(sol, st):
     return F, R, H, flag, fail, D, W
 
 
-def perturbed(case, st, kind):
+def perturbed(case, st, kind, seed=7):
     st = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in st.items()}
     if kind == "ulp":
         i = int(np.argmax(np.abs(st["u"][:, 2])))
         st["x"][i, 2] = np.nextafter(st["x"][i, 2], np.inf)
         st["u"][i, 2] = st["x"][i, 2] - case.X[i, 2]
     else:
-        sgn = np.where(np.random.default_rng(7).random(st["x"].shape) < 0.5, -1.0, 1.0)
+        sgn = np.where(np.random.default_rng(seed).random(st["x"].shape) < 0.5, -1.0, 1.0)
         st["x"] = np.asfortranarray(st["x"] * (1 + sgn * 2.0 ** -24))
         st["u"] = np.asfortranarray(st["x"] - case.X)
     return st
@@ -282,9 +282,11 @@ def floor_run(ref, st, n_incs, max_steps):
     return np.array(loads[:n_incs])
 
 
-def precision_study(case, ref, mix, max_steps):
+def precision_study(case, ref, mix, max_steps, floor_samples=0):
     """FP32-mixed against FP64 on the load history (the load is the penetrator family's static
-    reaction, calculatereactionforceFast.m:3-5, at the increment ends of the equilibrium staircase)."""
+    reaction, calculatereactionforceFast.m:3-5, at the increment ends of the equilibrium staircase).
+    `floor_samples` more FP64 runs with FP32-sized noise (other seeds) give the spread of FP64's own
+    peak under rounding-level perturbations."""
     t0 = time.time()
     ref.advance(lambda r: past_peak(r.loads()), max_steps)
     n = ref.n_incs()
@@ -324,6 +326,11 @@ def precision_study(case, ref, mix, max_steps):
     c_ulp = floor_run(ref, perturbed(case, first, "ulp"), n, max_steps)
     c_noise = floor_run(ref, perturbed(case, first, "noise"), n, max_steps)
     err = float((cm[pm] - peak) / peak)
+    extra = []
+    for seed in range(floor_samples):
+        c = floor_run(ref, perturbed(case, first, "noise", seed=100 + seed), n, max_steps)
+        extra.append(float((c.max() - peak) / peak))
+        log(f"  FP32-sized floor, seed {100 + seed}: peak {c.max():.2f} N ({extra[-1]:+.2e})")
     out = dict(
         protocol=None,
         load="static reaction of the penetrator family at the load-increment ends (converged states)",
@@ -340,7 +347,13 @@ def precision_study(case, ref, mix, max_steps):
         curve_fp64_fp32_sized=[round(float(v), 3) for v in c_noise],
         increment_end_steps_fp64=[int(x) for x in curve_of(ref.hist())[0][:n]],
         end_state=end_state, same_state_peak_load_err=near["load_err"], same_state=same,
+        **(dict(fp64_fp32_sized_floor_samples=extra) if floor_samples else {}),
         gate={"peak_load_err_max": GATE, "passed": abs(err) <= GATE})
+    if floor_samples:
+        # every FP64 run's peak against the unperturbed one (0), the mixed form's beside them
+        runs = [0.0, out["fp64_rounding_floor"], out["fp64_fp32_sized_floor"]] + extra
+        out["fp64_peak_err_range"] = [min(runs), max(runs)]
+        out["mixed_within_fp64_range"] = min(runs) <= err <= max(runs)
     log(f"  precision study {time.time() - t0:.0f}s: peak err {err:+.2e}, floors ulp "
         f"{out['fp64_rounding_floor']:+.2e}, FP32-sized {out['fp64_fp32_sized_floor']:+.2e}")
     return out, samples
@@ -638,6 +651,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-mixed vs FP64 load study")
+    ap.add_argument("--floor-samples", type=int, default=0,
+                    help="precision study: this many more FP64 runs under FP32-sized noise (the spread of FP64's peak)")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both",
                     help="which precision forms to run (a counter run runs one)")
     ap.add_argument("--small", action="store_true", help="dx = 2.5 mm notched trilinear beam instead")
@@ -751,7 +766,7 @@ def main():
     log(f"mixed: {rm['ms_per_step']:.3f} ms/step, bond kernel {rm['kernel_ms']:.3f} ms")
     prec, samples = None, []
     if not args.no_precision:
-        prec, samples = precision_study(case, fp64, mixed, args.max_study_steps)
+        prec, samples = precision_study(case, fp64, mixed, args.max_study_steps, args.floor_samples)
         prec["protocol"] = protocol_text(args)
     # a state inside the timed window for the counter runs: the FP64 checkpoint nearest to the
     # window's middle, or none (the counter run then steps from 0 as the window did)
